@@ -1,0 +1,179 @@
+"""Benchmark: bit-exact HMult + relinearize + rescale throughput on one MI355X.
+
+Workload (the plain GPU key-switching configuration): N = 2^16, L = 24 q
+limbs (the largest primes below 2^31 with q = 1 mod 2^17), alpha = 8
+special primes, dnum = 3; two ciphertexts of 2 polynomials with uniform
+residues and a uniform evaluation key, all resident in HBM before timing.
+One step = one Context::HMult with the merged RelinearizeRescale: Tensor ->
+ModUp (INTT, BaseConv, NTT) -> key inner product (+ P * (d0, d1)) ->
+ModDownAndRescale of both polynomials (INTT, BaseConv, NTT + epilogue).
+
+Prints one JSON line (see the "roofline" and "cpu_baseline" objects).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(REPO, "cheddar-fhe_amd")
+if PKG not in sys.path:
+    sys.path.insert(0, PKG)
+
+HBM_PEAK_GBS = 8000.0
+LEVEL = 23
+PARAMS = "rns24_aux8.json"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def uniform_poly(torch, ctx, pidx, gen):
+    """Uniform residues in [0, q_i) for the listed global prime indices."""
+    n = ctx.n
+    out = torch.empty((len(pidx), n), dtype=torch.int32, device="cuda")
+    for t, p in enumerate(pidx):
+        q = ctx.prime(p)
+        out[t] = torch.randint(0, q, (n,), dtype=torch.int64, device="cuda",
+                               generator=gen).to(torch.int32)
+    return out
+
+
+def cpu_baseline(prm, seconds=10.0):
+    """The CPU oracle's HMult at the same shape on the host cores (bounded
+    sample: as many HMults as fit in about `seconds`, at least 2)."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import numpy as np
+    import ckks_oracle as O
+
+    cores = min(16, os.cpu_count() or 1)
+    O.set_threads(cores)
+    P = O.Param.from_json(prm)
+    pidx = P.level_primes(LEVEL)
+    seed = 0x4348454444415200 + 1
+    ct1 = [P.uniform(pidx, seed + i) for i in range(2)]
+    ct2 = [P.uniform(pidx, seed + 2 + i) for i in range(2)]
+    allp = list(range(P.num_primes))
+    key = (np.stack([P.uniform(allp, seed + 10 + g) for g in range(P.dnum)]),
+           np.stack([P.uniform(allp, seed + 20 + g) for g in range(P.dnum)]))
+    P.hmult(LEVEL, ct1, ct2, key)  # warm-up
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        P.hmult(LEVEL, ct1, ct2, key)
+        n += 1
+        el = time.perf_counter() - t0
+        if (el >= seconds and n >= 2) or el >= 3 * seconds:
+            break
+    return {"value": n / el, "unit": "HMult/s", "cores": cores, "kind": "port",
+            "sample": f"{n} HMult+relin+rescale at L=24 on {cores} host threads "
+                      f"({el:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported "
+              "(the key-switching path has no multi-GPU form here)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import cheddar_amd as ck
+
+    prm = ck.load_param_json(PARAMS)
+    ctx = ck.Context.from_json(prm)
+    torch.cuda.set_device(0)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0x4348454444415201)
+    nm, nt = ctx.level_np(LEVEL)
+    qp = list(range(ctx.max_ter - nt, ctx.max_ter + nm))  # [ter reversed | main]
+    allp = list(range(ctx.num_primes))
+    log(f"context: L={ctx.L} alpha={ctx.alpha} dnum={ctx.dnum} level={LEVEL}")
+    ct1 = [uniform_poly(torch, ctx, qp, gen) for _ in range(2)]
+    ct2 = [uniform_poly(torch, ctx, qp, gen) for _ in range(2)]
+    key = ([uniform_poly(torch, ctx, allp, gen) for _ in range(ctx.dnum)],
+           [uniform_poly(torch, ctx, allp, gen) for _ in range(ctx.dnum)])
+    out = [torch.empty((ctx.num_q(LEVEL - 1), ctx.n), dtype=torch.int32, device="cuda")
+           for _ in range(2)]
+    stream = torch.cuda.current_stream()
+
+    def step():
+        ctx.hmult(out, ct1, ct2, key, LEVEL, rescale=True, stream=stream)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"warmup done ({a.warmup} steps)")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(a.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / a.steps
+    log(f"timed {a.steps} steps: {ms * 1e3:.1f} us/step (wall {wall * 1e3 / a.steps:.1f} us)")
+
+    # Dominant kernel family: the forward NTT (column + row passes), timed
+    # live on the same stream over the extended basis (32 limbs).
+    x = uniform_poly(torch, ctx, allp, gen)
+    y = torch.empty_like(x)
+    reps = 50
+    for _ in range(3):
+        ctx.ntt(y, x, ctx.max_main, ctx.max_ter, ctx.alpha, stream=stream)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        ctx.ntt(y, x, ctx.max_main, ctx.max_ter, ctx.alpha, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ntt_s = e0.elapsed_time(e1) / reps / 1e3
+    limbs = x.shape[0]
+    alg_bytes = limbs * ctx.n * 8  # read once + write once per coefficient
+    achieved = alg_bytes / ntt_s / 1e9
+    log(f"forward NTT of {limbs} limbs: {ntt_s * 1e6:.1f} us, {achieved:.0f} GB/s")
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        log("cpu baseline (oracle) ...")
+        cpu = cpu_baseline(prm)
+        log(f"cpu baseline: {cpu['value']:.3f} HMult/s")
+
+    res = {
+        "metric": "hmult_relin_rescale_per_s",
+        "value": 1e3 / ms,
+        "unit": "HMult/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic uniform residues and evaluation key (seeded)",
+        "config": {"workload": "HMult+relin+rescale, N=2^16, L=24, alpha=8, dnum=3, "
+                               "plain HIP form", "log_n": 16, "L": ctx.L, "alpha": ctx.alpha,
+                   "dnum": ctx.dnum, "level": LEVEL, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "ntt_fwd_cols+ntt_fwd_rows (one forward NTT)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "launch_us": ntt_s * 1e6, "alg_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,295 @@
+"""Python binding of libcheddar_amd.so (the C ABI in include/cheddar_amd.h).
+
+PyTorch provides device memory and streams here; every computation runs in
+the HIP kernels of libcheddar_amd.so. Importing this module on a machine
+without the built library raises: there is no CPU fallback.
+
+Polynomials are torch int32 tensors on the GPU holding the u32 residues bit
+for bit, shaped [limbs, N] (limb-major, the reference's layout).
+"""
+import ctypes
+import json
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcheddar_amd.so")
+PARAM_DIR = os.path.join(HERE, "params")
+
+CK_MODDOWN, CK_RESCALE, CK_MODDOWN_RESCALE = 0, 1, 2
+
+_vp = ctypes.c_void_p
+_ip = ctypes.POINTER(ctypes.c_int)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+_SIGS = {
+    "ck_last_error": (ctypes.c_char_p, []),
+    "ck_version": (ctypes.c_int, []),
+    "ck_context_create": (ctypes.c_int, [ctypes.c_int, _u32p, ctypes.c_int, _u32p, ctypes.c_int,
+                                          _u32p, ctypes.c_int, _ip, ctypes.c_int,
+                                          ctypes.POINTER(_vp)]),
+    "ck_context_destroy": (ctypes.c_int, [_vp]),
+    "ck_context_info": (ctypes.c_int, [_vp, _ip]),
+    "ck_level_to_np": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip]),
+    "ck_np_to_level": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "ck_modup_beta": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "ck_galois_factor": (ctypes.c_int, [_vp, ctypes.c_int, _u32p]),
+    "ck_prime": (ctypes.c_int, [_vp, ctypes.c_int, _u32p]),
+    "ck_ntt": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_int, _vp]),
+    "ck_intt": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, _vp]),
+    "ck_intt_mult_const": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, _u32p, ctypes.c_int, _vp]),
+    "ck_modup": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp]),
+    "ck_moddown": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
+    "ck_add": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
+    "ck_sub": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
+    "ck_mult": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
+    "ck_neg": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
+    "ck_mult_const": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, _vp]),
+    "ck_tensor": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int] + [_vp] * 7 + [_vp]),
+    "ck_permute": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, _vp]),
+    "ck_mult_key_no_moddown": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
+                                              ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
+    "ck_mult_key": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp,
+                                   ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
+    "ck_relinearize": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
+                                      ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, _vp]),
+    "ck_hmult": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, _vp]),
+    "ck_hrot": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp,
+                               ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, _vp]),
+    "ck_hconj": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp,
+                                ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
+    "ck_rescale": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp),
+                                  ctypes.POINTER(_vp), _vp]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+
+def load_library(path=LIB_PATH):
+    """Load libcheddar_amd.so and declare every entry point's signature."""
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built: run python cheddar-fhe_amd/build.py")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = load_library()
+
+
+class CheddarError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise CheddarError(f"status {rc}: {lib.ck_last_error().decode()}")
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _ptrs(ts):
+    arr = (ctypes.c_void_p * max(1, len(ts)))()
+    for i, t in enumerate(ts):
+        arr[i] = None if t is None else t.data_ptr()
+    return arr
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def load_param_json(path_or_name):
+    p = path_or_name
+    if not os.path.exists(p):
+        p = os.path.join(PARAM_DIR, path_or_name)
+    with open(p) as f:
+        return json.load(f)
+
+
+class Context:
+    """Owns a ck_context: the reference's Parameter + Context pair
+    (include/core/Parameter.h:42-47, include/core/Context.h:70)."""
+
+    def __init__(self, log_degree, main_primes, aux_primes, ter_primes=(), level_config=None):
+        main = (ctypes.c_uint32 * max(1, len(main_primes)))(*main_primes)
+        ter = (ctypes.c_uint32 * max(1, len(ter_primes)))(*ter_primes)
+        aux = (ctypes.c_uint32 * max(1, len(aux_primes)))(*aux_primes)
+        if level_config is None:
+            level_config = [[i + 1, 0] for i in range(len(main_primes))]
+        flat = [int(v) for pair in level_config for v in pair]
+        cfg = (ctypes.c_int * len(flat))(*flat)
+        h = ctypes.c_void_p()
+        _check(lib.ck_context_create(int(log_degree), main, len(main_primes),
+                                     ter if len(ter_primes) else None, len(ter_primes),
+                                     aux, len(aux_primes), cfg, len(level_config),
+                                     ctypes.byref(h)))
+        self._h = h
+        info = (ctypes.c_int * 10)()
+        _check(lib.ck_context_info(self._h, info))
+        (self.log_n, self.n, self.max_main, self.max_ter, self.alpha, self.L, self.dnum,
+         self.max_level, self.num_primes, self.workspace_mib) = list(info)
+        self.level_config = [tuple(x) for x in level_config]
+
+    @classmethod
+    def from_json(cls, d):
+        if isinstance(d, str):
+            d = load_param_json(d)
+        return cls(d["log_degree"], d["main_primes"], d["auxiliary_primes"],
+                   d.get("terminal_primes", []), d["level_config"])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.ck_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- parameters -----------------------------------------------------
+    def level_np(self, level):
+        m, t = ctypes.c_int(), ctypes.c_int()
+        _check(lib.ck_level_to_np(self._h, level, ctypes.byref(m), ctypes.byref(t)))
+        return m.value, t.value
+
+    def num_q(self, level):
+        m, t = self.level_np(level)
+        return m + t
+
+    def np_to_level(self, num_main, num_ter):
+        return lib.ck_np_to_level(self._h, num_main, num_ter)
+
+    def modup_beta(self, level):
+        return lib.ck_modup_beta(self._h, level)
+
+    def galois_factor(self, rot):
+        g = ctypes.c_uint32()
+        _check(lib.ck_galois_factor(self._h, rot, ctypes.byref(g)))
+        return g.value
+
+    def prime(self, pidx):
+        g = ctypes.c_uint32()
+        _check(lib.ck_prime(self._h, pidx, ctypes.byref(g)))
+        return g.value
+
+    # ---- NTTHandler -----------------------------------------------------
+    def ntt(self, dst, src, num_main, num_ter, num_aux=0, src_gap=0, mont_conv=False, stream=None):
+        _check(lib.ck_ntt(self._h, _ptr(dst), _ptr(src), num_main, num_ter, num_aux, src_gap,
+                          int(mont_conv), _stream(stream)))
+
+    def intt(self, dst, src, num_main, num_ter, num_aux=0, src_gap=0, mont_conv=True, stream=None):
+        _check(lib.ck_intt(self._h, _ptr(dst), _ptr(src), num_main, num_ter, num_aux, src_gap,
+                           int(mont_conv), _stream(stream)))
+
+    def intt_mult_const(self, dst, src, num_main, num_ter, num_aux, consts, normalize,
+                        src_gap=0, stream=None):
+        c = (ctypes.c_uint32 * len(consts))(*[int(v) for v in consts])
+        _check(lib.ck_intt_mult_const(self._h, _ptr(dst), _ptr(src), num_main, num_ter, num_aux,
+                                      src_gap, c, int(normalize), _stream(stream)))
+
+    # ---- ModSwitchHandler -------------------------------------------------
+    def modup(self, dst, src, level, stream=None):
+        _check(lib.ck_modup(self._h, level, _ptr(dst), _ptr(src), _stream(stream)))
+
+    def moddown(self, dsts, srcs, level, kind, stream=None):
+        _check(lib.ck_moddown(self._h, level, kind, len(dsts), _ptrs(dsts), _ptrs(srcs),
+                              _stream(stream)))
+
+    # ---- ElementWiseHandler ----------------------------------------------
+    def add(self, dsts, xs, ys, np3, stream=None):
+        _check(lib.ck_add(self._h, *np3, len(dsts), _ptrs(dsts), _ptrs(xs), _ptrs(ys),
+                          _stream(stream)))
+
+    def sub(self, dsts, xs, ys, np3, stream=None):
+        _check(lib.ck_sub(self._h, *np3, len(dsts), _ptrs(dsts), _ptrs(xs), _ptrs(ys),
+                          _stream(stream)))
+
+    def mult(self, dsts, xs, ys, np3, stream=None):
+        _check(lib.ck_mult(self._h, *np3, len(dsts), _ptrs(dsts), _ptrs(xs), _ptrs(ys),
+                           _stream(stream)))
+
+    def neg(self, dsts, xs, np3, stream=None):
+        _check(lib.ck_neg(self._h, *np3, len(dsts), _ptrs(dsts), _ptrs(xs), _stream(stream)))
+
+    def mult_const(self, dsts, xs, consts_dev, np3, stream=None):
+        _check(lib.ck_mult_const(self._h, *np3, len(dsts), _ptrs(dsts), _ptrs(xs),
+                                 _ptr(consts_dev), _stream(stream)))
+
+    def tensor(self, d0, d1, d2, b1, a1, b2, a2, num_main, num_ter, stream=None):
+        _check(lib.ck_tensor(self._h, num_main, num_ter, _ptr(d0), _ptr(d1), _ptr(d2),
+                             _ptr(b1), _ptr(a1), _ptr(b2), _ptr(a2), _stream(stream)))
+
+    def permute(self, dsts, xs, rot, np3, stream=None):
+        _check(lib.ck_permute(self._h, *np3, len(dsts), _ptrs(dsts), _ptrs(xs), rot,
+                              _stream(stream)))
+
+    # ---- Context evaluator -------------------------------------------------
+    def mult_key_no_moddown(self, acc_b, acc_a, x, key, level, add_b=None, add_a=None,
+                            stream=None):
+        _check(lib.ck_mult_key_no_moddown(self._h, level, _ptr(acc_b), _ptr(acc_a), _ptr(x),
+                                          _ptr(add_b), _ptr(add_a), _ptrs(key[0]), _ptrs(key[1]),
+                                          _stream(stream)))
+
+    def mult_key(self, out, ct, key, level, stream=None):
+        _check(lib.ck_mult_key(self._h, level, _ptr(out[0]), _ptr(out[1]), _ptr(ct[0]),
+                               _ptr(ct[1]), _ptrs(key[0]), _ptrs(key[1]), _stream(stream)))
+
+    def relinearize(self, out, d, key, level, rescale=True, stream=None):
+        _check(lib.ck_relinearize(self._h, level, _ptr(out[0]), _ptr(out[1]), _ptr(d[0]),
+                                  _ptr(d[1]), _ptr(d[2]), _ptrs(key[0]), _ptrs(key[1]),
+                                  int(rescale), _stream(stream)))
+
+    def hmult(self, out, ct1, ct2, key, level, rescale=True, stream=None):
+        _check(lib.ck_hmult(self._h, level, _ptr(out[0]), _ptr(out[1]), _ptr(ct1[0]),
+                            _ptr(ct1[1]), _ptr(ct2[0]), _ptr(ct2[1]), _ptrs(key[0]),
+                            _ptrs(key[1]), int(rescale), _stream(stream)))
+
+    def hrot(self, out, ct, key, rot, level, stream=None):
+        _check(lib.ck_hrot(self._h, level, _ptr(out[0]), _ptr(out[1]), _ptr(ct[0]), _ptr(ct[1]),
+                           _ptrs(key[0]), _ptrs(key[1]), int(rot), _stream(stream)))
+
+    def hconj(self, out, ct, key, level, stream=None):
+        _check(lib.ck_hconj(self._h, level, _ptr(out[0]), _ptr(out[1]), _ptr(ct[0]), _ptr(ct[1]),
+                            _ptrs(key[0]), _ptrs(key[1]), _stream(stream)))
+
+    def rescale(self, dsts, srcs, level, stream=None):
+        _check(lib.ck_rescale(self._h, level, len(dsts), _ptrs(dsts), _ptrs(srcs),
+                              _stream(stream)))
+
+
+# ---- host <-> device helpers ------------------------------------------------
+def to_device(arr_u32, device="cuda"):
+    """numpy uint32 [limbs, N] -> torch int32 device tensor (same bits)."""
+    import numpy as np
+    import torch
+    a = np.ascontiguousarray(arr_u32, dtype=np.uint32).view(np.int32)
+    return torch.from_numpy(a).to(device)
+
+
+def to_host(t):
+    """torch int32 device tensor -> numpy uint32 (same bits)."""
+    import numpy as np
+    return t.detach().cpu().numpy().view(np.uint32)

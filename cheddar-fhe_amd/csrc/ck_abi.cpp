@@ -1,0 +1,415 @@
+// extern "C" entry points of include/cheddar_amd.h: argument validation
+// (the reference's AssertTrue checks, returned as status codes) and
+// dispatch to the host orchestration in ck_ops.cpp.
+#include "../../include/cheddar_amd.h"
+
+#include <cstring>
+
+#include "ck_ops.h"
+
+struct ck_context {
+  ck::Context *c;
+};
+
+using namespace ck;
+
+namespace {
+
+constexpr size_t kLimb = 65536;
+
+int fail(int code, const char *msg) {
+  set_error(msg);
+  return code;
+}
+
+#define CK_REQUIRE(cond, msg) \
+  do {                        \
+    if (!(cond)) return fail(CK_ERR_INVALID_ARG, msg); \
+  } while (0)
+
+hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+bool valid_np(const Context &c, int nm, int nt, int na) {
+  return nm >= 0 && nm <= c.max_main && nt >= 0 && nt <= c.max_ter &&
+         na >= 0 && na <= c.alpha && nm + nt + na > 0;
+}
+
+bool valid_level(const Context &c, int level) {
+  return level >= 0 && level <= c.max_level;
+}
+
+bool valid_ptrs(const uint32_t *const *p, int n) {
+  if (!p) return false;
+  for (int i = 0; i < n; i++)
+    if (!p[i]) return false;
+  return true;
+}
+
+bool valid_key(const Context &c, const uint32_t *const *k, int level) {
+  if (!k) return false;
+  const LevelPlan &lp = c.levels[level];
+  for (int g = 0; g < lp.beta; g++)
+    if (lp.groups[g].s1 > lp.groups[g].s0 && !k[g]) return false;
+  return true;
+}
+
+int elementwise(ck_context *ctx, int op, int nm, int nt, int na, int npoly,
+                uint32_t *const *dst, const uint32_t *const *x,
+                const uint32_t *const *y, void *stream) {
+  CK_REQUIRE(ctx && ctx->c, "null context");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_np(c, nm, nt, na), "invalid NP");
+  CK_REQUIRE(npoly >= 1 && npoly <= kMaxPolys, "npoly must be 1..3");
+  CK_REQUIRE(valid_ptrs(dst, npoly) && valid_ptrs(x, npoly), "null pointer");
+  EwNp np = make_np(c, nm, nt, na);
+  EwOperand ox{}, oy{}, od{};
+  for (int j = 0; j < npoly; j++) {
+    ox.ptr[j] = const_cast<uint32_t *>(x[j]);
+    od.ptr[j] = dst[j];
+  }
+  if (op <= 2) {
+    CK_REQUIRE(valid_ptrs(y, npoly), "null pointer");
+    for (int j = 0; j < npoly; j++) oy.ptr[j] = const_cast<uint32_t *>(y[j]);
+    launch_binary(c, op, np, ox, oy, od, npoly, as_stream(stream));
+  } else {
+    launch_unary(c, 0, np, ox, od, npoly, as_stream(stream));
+  }
+  return check_hip(hipGetLastError(), "elementwise");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ck_last_error(void) { return ck::last_error(); }
+
+int ck_version(void) { return 100; }
+
+int ck_context_create(int log_degree, const uint32_t *main_primes, int n_main,
+                      const uint32_t *ter_primes, int n_ter,
+                      const uint32_t *aux_primes, int n_aux,
+                      const int *level_config, int n_levels,
+                      ck_context **out) {
+  CK_REQUIRE(out, "out is null");
+  *out = nullptr;
+  Context *c = nullptr;
+  int rc = context_create(log_degree, main_primes, n_main, ter_primes, n_ter,
+                          aux_primes, n_aux, level_config, n_levels, &c);
+  if (rc) return rc;
+  *out = new ck_context{c};
+  return CK_OK;
+}
+
+int ck_context_destroy(ck_context *ctx) {
+  if (!ctx) return CK_OK;
+  context_destroy(ctx->c);
+  delete ctx;
+  return CK_OK;
+}
+
+int ck_context_info(const ck_context *ctx, int *out) {
+  CK_REQUIRE(ctx && ctx->c && out, "null argument");
+  const Context &c = *ctx->c;
+  out[0] = c.log_n;
+  out[1] = c.n;
+  out[2] = c.max_main;
+  out[3] = c.max_ter;
+  out[4] = c.alpha;
+  out[5] = c.L;
+  out[6] = c.dnum;
+  out[7] = c.max_level;
+  out[8] = c.num_primes;
+  out[9] = (int)((c.ws_words * sizeof(uint32_t)) >> 20);
+  return CK_OK;
+}
+
+int ck_level_to_np(const ck_context *ctx, int level, int *num_main,
+                   int *num_ter) {
+  CK_REQUIRE(ctx && ctx->c && num_main && num_ter, "null argument");
+  CK_REQUIRE(valid_level(*ctx->c, level), "invalid level");
+  *num_main = ctx->c->lv_main[level];
+  *num_ter = ctx->c->lv_ter[level];
+  return CK_OK;
+}
+
+int ck_np_to_level(const ck_context *ctx, int num_main, int num_ter) {
+  if (!ctx || !ctx->c) return -1;
+  const Context &c = *ctx->c;
+  for (int l = 0; l <= c.max_level; l++)
+    if (c.lv_main[l] == num_main && c.lv_ter[l] == num_ter) return l;
+  return -1;
+}
+
+int ck_modup_beta(const ck_context *ctx, int level) {
+  if (!ctx || !ctx->c || !valid_level(*ctx->c, level)) return -1;
+  return ctx->c->levels[level].beta;
+}
+
+int ck_galois_factor(const ck_context *ctx, int rot, uint32_t *out) {
+  CK_REQUIRE(ctx && ctx->c && out, "null argument");
+  const Context &c = *ctx->c;
+  if (rot == -1) {
+    *out = 2u * c.n - 1;
+    return CK_OK;
+  }
+  CK_REQUIRE(rot >= 0 && rot <= c.n / 2, "invalid permute amount");
+  *out = c.galois[rot];
+  return CK_OK;
+}
+
+int ck_prime(const ck_context *ctx, int pidx, uint32_t *out) {
+  CK_REQUIRE(ctx && ctx->c && out, "null argument");
+  CK_REQUIRE(pidx >= 0 && pidx < ctx->c->num_primes, "invalid prime index");
+  *out = ctx->c->primes[pidx];
+  return CK_OK;
+}
+
+int ck_ntt(ck_context *ctx, uint32_t *dst, const uint32_t *src, int num_main,
+           int num_ter, int num_aux, int src_gap, int montgomery_conversion,
+           void *stream) {
+  CK_REQUIRE(ctx && ctx->c && dst && src, "null argument");
+  CK_REQUIRE(valid_np(*ctx->c, num_main, num_ter, num_aux), "invalid NP");
+  CK_REQUIRE(src_gap >= 0 && (src_gap == 0 || src != dst),
+             "a limb gap requires dst != src");
+  return op_ntt(*ctx->c, dst, src, num_main, num_ter, num_aux, src_gap,
+                montgomery_conversion != 0, as_stream(stream));
+}
+
+int ck_intt(ck_context *ctx, uint32_t *dst, const uint32_t *src, int num_main,
+            int num_ter, int num_aux, int src_gap, int montgomery_conversion,
+            void *stream) {
+  CK_REQUIRE(ctx && ctx->c && dst && src, "null argument");
+  CK_REQUIRE(valid_np(*ctx->c, num_main, num_ter, num_aux), "invalid NP");
+  CK_REQUIRE(src_gap >= 0 && (src_gap == 0 || src != dst),
+             "a limb gap requires dst != src");
+  return op_intt(*ctx->c, dst, src, num_main, num_ter, num_aux, src_gap,
+                 montgomery_conversion != 0, as_stream(stream));
+}
+
+int ck_intt_mult_const(ck_context *ctx, uint32_t *dst, const uint32_t *src,
+                       int num_main, int num_ter, int num_aux, int src_gap,
+                       const uint32_t *consts, int normalize, void *stream) {
+  CK_REQUIRE(ctx && ctx->c && dst && src && consts, "null argument");
+  CK_REQUIRE(valid_np(*ctx->c, num_main, num_ter, num_aux), "invalid NP");
+  CK_REQUIRE(src_gap >= 0 && (src_gap == 0 || src != dst),
+             "a limb gap requires dst != src");
+  return op_intt_const(*ctx->c, dst, src, num_main, num_ter, num_aux, src_gap,
+                       consts, normalize != 0, as_stream(stream));
+}
+
+int ck_modup(ck_context *ctx, int level, uint32_t *dst, const uint32_t *src,
+             void *stream) {
+  CK_REQUIRE(ctx && ctx->c && dst && src, "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_level(c, level), "invalid level");
+  CK_REQUIRE(dst != src, "ModUp is not in place");
+  return op_modup(c, level, dst, src, c.ws, as_stream(stream));
+}
+
+int ck_moddown(ck_context *ctx, int level, int kind, int npoly,
+               uint32_t *const *dst, const uint32_t *const *src,
+               void *stream) {
+  CK_REQUIRE(ctx && ctx->c, "null context");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_level(c, level), "invalid level");
+  CK_REQUIRE(kind >= 0 && kind <= 2, "invalid ModDown kind");
+  CK_REQUIRE(npoly >= 1 && npoly <= 4, "npoly must be 1..4");
+  CK_REQUIRE(valid_ptrs(dst, npoly) && valid_ptrs(src, npoly), "null pointer");
+  for (int j = 0; j < npoly; j++)
+    for (int k = 0; k < npoly; k++)
+      CK_REQUIRE(dst[j] != src[k], "ModDown dst and src must differ");
+  CK_REQUIRE(moddown_plan(c, level, kind), "no such rescale at this level");
+  return op_moddown(c, level, kind, npoly, dst, src, c.ws, as_stream(stream));
+}
+
+int ck_add(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly,
+           uint32_t *const *dst, const uint32_t *const *x,
+           const uint32_t *const *y, void *stream) {
+  return elementwise(ctx, 0, num_main, num_ter, num_aux, npoly, dst, x, y,
+                     stream);
+}
+
+int ck_sub(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly,
+           uint32_t *const *dst, const uint32_t *const *x,
+           const uint32_t *const *y, void *stream) {
+  return elementwise(ctx, 1, num_main, num_ter, num_aux, npoly, dst, x, y,
+                     stream);
+}
+
+int ck_mult(ck_context *ctx, int num_main, int num_ter, int num_aux,
+            int npoly, uint32_t *const *dst, const uint32_t *const *x,
+            const uint32_t *const *y, void *stream) {
+  return elementwise(ctx, 2, num_main, num_ter, num_aux, npoly, dst, x, y,
+                     stream);
+}
+
+int ck_neg(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly,
+           uint32_t *const *dst, const uint32_t *const *x, void *stream) {
+  return elementwise(ctx, 3, num_main, num_ter, num_aux, npoly, dst, x,
+                     nullptr, stream);
+}
+
+int ck_mult_const(ck_context *ctx, int num_main, int num_ter, int num_aux,
+                  int npoly, uint32_t *const *dst, const uint32_t *const *x,
+                  const uint32_t *consts, void *stream) {
+  CK_REQUIRE(ctx && ctx->c && consts, "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_np(c, num_main, num_ter, num_aux), "invalid NP");
+  CK_REQUIRE(npoly >= 1 && npoly <= kMaxPolys, "npoly must be 1..3");
+  CK_REQUIRE(valid_ptrs(dst, npoly) && valid_ptrs(x, npoly), "null pointer");
+  EwNp np = make_np(c, num_main, num_ter, num_aux);
+  EwOperand ox{}, od{};
+  for (int j = 0; j < npoly; j++) {
+    ox.ptr[j] = const_cast<uint32_t *>(x[j]);
+    od.ptr[j] = dst[j];
+  }
+  launch_const(c, 0, np, ox, consts, 0, od, npoly, as_stream(stream));
+  return check_hip(hipGetLastError(), "mult_const");
+}
+
+int ck_tensor(ck_context *ctx, int num_main, int num_ter, uint32_t *d0,
+              uint32_t *d1, uint32_t *d2, const uint32_t *b1,
+              const uint32_t *a1, const uint32_t *b2, const uint32_t *a2,
+              void *stream) {
+  CK_REQUIRE(ctx && ctx->c && d0 && d1 && d2 && b1 && a1 && b2 && a2,
+             "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_np(c, num_main, num_ter, 0), "invalid NP");
+  EwNp np = make_np(c, num_main, num_ter, 0);
+  EwOperand x{{const_cast<uint32_t *>(b1), const_cast<uint32_t *>(a1), nullptr},
+              0};
+  EwOperand y{{const_cast<uint32_t *>(b2), const_cast<uint32_t *>(a2), nullptr},
+              0};
+  EwOperand o{{d0, d1, d2}, 0};
+  launch_tensor(c, np, x, y, o, as_stream(stream));
+  return check_hip(hipGetLastError(), "tensor");
+}
+
+int ck_permute(ck_context *ctx, int num_main, int num_ter, int num_aux,
+               int npoly, uint32_t *const *dst, const uint32_t *const *x,
+               int rot, void *stream) {
+  CK_REQUIRE(ctx && ctx->c, "null context");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_np(c, num_main, num_ter, num_aux), "invalid NP");
+  CK_REQUIRE(npoly >= 1 && npoly <= kMaxPolys, "npoly must be 1..3");
+  CK_REQUIRE(valid_ptrs(dst, npoly) && valid_ptrs(x, npoly), "null pointer");
+  for (int j = 0; j < npoly; j++)
+    CK_REQUIRE(dst[j] != x[j], "Permute does not support in-place operation");
+  uint32_t g = 0;
+  int rc = ck_galois_factor(ctx, rot, &g);
+  if (rc) return rc;
+  EwNp np = make_np(c, num_main, num_ter, num_aux);
+  EwOperand ox{}, od{};
+  for (int j = 0; j < npoly; j++) {
+    ox.ptr[j] = const_cast<uint32_t *>(x[j]);
+    od.ptr[j] = dst[j];
+  }
+  launch_permute(c, np, ox, od, g, npoly, as_stream(stream));
+  return check_hip(hipGetLastError(), "permute");
+}
+
+int ck_mult_key_no_moddown(ck_context *ctx, int level, uint32_t *accum_b,
+                           uint32_t *accum_a, const uint32_t *x,
+                           const uint32_t *add_b, const uint32_t *add_a,
+                           const uint32_t *const *key_b,
+                           const uint32_t *const *key_a, void *stream) {
+  CK_REQUIRE(ctx && ctx->c && accum_b && accum_a && x, "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_level(c, level), "invalid level");
+  CK_REQUIRE(valid_key(c, key_b, level) && valid_key(c, key_a, level),
+             "null key block");
+  CK_REQUIRE(accum_b != x && accum_a != x,
+             "in-place operation is not supported for MultKeyNoModDown");
+  return op_mult_key_no_moddown(c, level, accum_b, accum_a, x, add_b, add_a,
+                                key_b, key_a, c.ws, as_stream(stream));
+}
+
+int ck_mult_key(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
+                const uint32_t *b, const uint32_t *a,
+                const uint32_t *const *key_b, const uint32_t *const *key_a,
+                void *stream) {
+  CK_REQUIRE(ctx && ctx->c && out_b && out_a && b && a, "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_level(c, level), "invalid level");
+  CK_REQUIRE(valid_key(c, key_b, level) && valid_key(c, key_a, level),
+             "null key block");
+  return op_mult_key(c, level, out_b, out_a, b, a, key_b, key_a, c.ws,
+                     as_stream(stream));
+}
+
+int ck_relinearize(ck_context *ctx, int level, uint32_t *out_b,
+                   uint32_t *out_a, const uint32_t *d0, const uint32_t *d1,
+                   const uint32_t *d2, const uint32_t *const *key_b,
+                   const uint32_t *const *key_a, int rescale, void *stream) {
+  CK_REQUIRE(ctx && ctx->c && out_b && out_a && d0 && d1 && d2,
+             "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_level(c, level), "invalid level");
+  CK_REQUIRE(!rescale || moddown_plan(c, level, 2),
+             "cannot rescale at this level");
+  CK_REQUIRE(valid_key(c, key_b, level) && valid_key(c, key_a, level),
+             "null key block");
+  return op_relinearize(c, level, out_b, out_a, d0, d1, d2, key_b, key_a,
+                        rescale != 0, c.ws, as_stream(stream));
+}
+
+int ck_hmult(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
+             const uint32_t *b1, const uint32_t *a1, const uint32_t *b2,
+             const uint32_t *a2, const uint32_t *const *key_b,
+             const uint32_t *const *key_a, int rescale, void *stream) {
+  CK_REQUIRE(ctx && ctx->c && out_b && out_a && b1 && a1 && b2 && a2,
+             "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_level(c, level), "invalid level");
+  CK_REQUIRE(!rescale || moddown_plan(c, level, 2),
+             "cannot rescale at this level");
+  CK_REQUIRE(valid_key(c, key_b, level) && valid_key(c, key_a, level),
+             "null key block");
+  return op_hmult(c, level, out_b, out_a, b1, a1, b2, a2, key_b, key_a,
+                  rescale != 0, as_stream(stream));
+}
+
+int ck_hrot(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
+            const uint32_t *b, const uint32_t *a, const uint32_t *const *key_b,
+            const uint32_t *const *key_a, int rot, void *stream) {
+  CK_REQUIRE(ctx && ctx->c && out_b && out_a && b && a, "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_level(c, level), "invalid level");
+  const int slots = c.n / 2;
+  rot %= slots;
+  if (rot < 0) rot += slots;
+  const size_t bytes = (size_t)c.levels[level].num_q * kLimb * 4;
+  if (rot == 0) {
+    // Context::HRot copies when the rotation is a no-op
+    int rc = check_hip(hipMemcpyAsync(out_b, b, bytes, hipMemcpyDeviceToDevice,
+                                      as_stream(stream)),
+                       "hrot copy");
+    if (rc) return rc;
+    return check_hip(hipMemcpyAsync(out_a, a, bytes, hipMemcpyDeviceToDevice,
+                                    as_stream(stream)),
+                     "hrot copy");
+  }
+  CK_REQUIRE(valid_key(c, key_b, level) && valid_key(c, key_a, level),
+             "null key block");
+  return op_hrot_galois(c, level, out_b, out_a, b, a, key_b, key_a,
+                        c.galois[rot], as_stream(stream));
+}
+
+int ck_hconj(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
+             const uint32_t *b, const uint32_t *a, const uint32_t *const *key_b,
+             const uint32_t *const *key_a, void *stream) {
+  CK_REQUIRE(ctx && ctx->c && out_b && out_a && b && a, "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_level(c, level), "invalid level");
+  CK_REQUIRE(valid_key(c, key_b, level) && valid_key(c, key_a, level),
+             "null key block");
+  return op_hrot_galois(c, level, out_b, out_a, b, a, key_b, key_a,
+                        2u * c.n - 1, as_stream(stream));
+}
+
+int ck_rescale(ck_context *ctx, int level, int npoly, uint32_t *const *dst,
+               const uint32_t *const *src, void *stream) {
+  return ck_moddown(ctx, level, CK_RESCALE, npoly, dst, src, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,126 @@
+// Device-side modular arithmetic for 32-bit RNS residues (q < 2^31) on
+// CDNA4 (gfx950).
+//
+// Two exact arithmetic families are provided:
+//   * signed lazy Montgomery with R = 2^32, the word-level convention the
+//     reference stores its data in (include/common/Basic.cuh:72-260):
+//     results of a product of stored values carry the factor R^-1;
+//   * exact FP64 modular products for transforms: a residue is held as a
+//     double (integers up to 2^53 are exact), a * w mod q is computed as
+//     hi = a*w, lo = fma(a, w, -hi) (the exact rounding error), k =
+//     rint(a * w/q), r = fma(-k, q, hi) + lo. FP64 FMA is full rate on
+//     CDNA4 while 32-bit integer multiplies are quarter rate, so transforms
+//     use this form. Both families produce the same canonical residues.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ck {
+
+// ---------------------------------------------------------------------------
+// Montgomery (R = 2^32), signed lazy, as Basic.cuh:90-99 defines it
+// ---------------------------------------------------------------------------
+
+// |a| <= q * 2^31  ->  a * 2^-32 mod q in (-q, q)
+__device__ __forceinline__ int32_t mont_reduce_lazy(int64_t a, uint32_t q,
+                                                    int32_t qinv) {
+  int32_t lo = (int32_t)(uint32_t)(uint64_t)a;
+  int32_t hi = (int32_t)(a >> 32);
+  int32_t t = lo * qinv;
+  int32_t m = __mulhi(t, (int32_t)q);
+  return hi - m;
+}
+
+__device__ __forceinline__ uint32_t mont_reduce(int64_t a, uint32_t q,
+                                                int32_t qinv) {
+  int32_t r = mont_reduce_lazy(a, q, qinv);
+  r += (int32_t)q & (r >> 31);
+  return (uint32_t)r;
+}
+
+// a, b in (-2^31, 2^31) with |a*b| <= q*2^31: a * b * 2^-32 mod q in (-q,q)
+__device__ __forceinline__ int32_t mont_mul_lazy(int32_t a, int32_t b,
+                                                 uint32_t q, int32_t qinv) {
+  return mont_reduce_lazy((int64_t)a * (int64_t)b, q, qinv);
+}
+
+// a, b in [0, q): a * b * 2^-32 mod q in [0, q)
+__device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t b,
+                                             uint32_t q, int32_t qinv) {
+  int32_t r = mont_mul_lazy((int32_t)a, (int32_t)b, q, qinv);
+  r += (int32_t)q & (r >> 31);
+  return (uint32_t)r;
+}
+
+// Montgomery product with a constant w whose companion wq = w * qinv mod
+// 2^32 is precomputed: three multiplies instead of four.
+__device__ __forceinline__ int32_t mont_mul_pre(int32_t a, int32_t w,
+                                                int32_t wq, uint32_t q) {
+  int32_t hi = __mulhi(a, w);
+  int32_t t = a * wq;
+  return hi - __mulhi(t, (int32_t)q);
+}
+
+__device__ __forceinline__ uint32_t add_mod(uint32_t a, uint32_t b,
+                                            uint32_t q) {
+  uint32_t r = a + b;
+  return r >= q ? r - q : r;
+}
+
+__device__ __forceinline__ uint32_t sub_mod(uint32_t a, uint32_t b,
+                                            uint32_t q) {
+  int32_t r = (int32_t)a - (int32_t)b;
+  r += (int32_t)q & (r >> 31);
+  return (uint32_t)r;
+}
+
+// [0, q) -> [-(q-1)/2, (q-1)/2]  (Basic.cuh:226-239)
+__device__ __forceinline__ int32_t centre(uint32_t a, uint32_t q) {
+  return a > (q >> 1) ? (int32_t)a - (int32_t)q : (int32_t)a;
+}
+
+// ---------------------------------------------------------------------------
+// Exact FP64 modular arithmetic
+// ---------------------------------------------------------------------------
+
+struct ModF {
+  double q;     // the prime
+  double qinv;  // 1/q rounded
+};
+
+// a * w mod q for integer-valued doubles; |a| < 2^48, 0 <= w < q < 2^31,
+// wq = w/q (any correctly rounded value). Result r == a*w (mod q) exactly,
+// |r| < 1.5 q + 2^26.
+__device__ __forceinline__ double mulmod_f(double a, double w, double wq,
+                                           double q) {
+  double hi = a * w;
+  double lo = __builtin_fma(a, w, -hi);
+  double k = __builtin_rint(a * wq);
+  return __builtin_fma(-k, q, hi) + lo;
+}
+
+// |x| < 2^50 -> x mod q in (-q/2 - 1, q/2 + 1) (one rounding step)
+__device__ __forceinline__ double reduce_f(double x, const ModF &m) {
+  double k = __builtin_rint(x * m.qinv);
+  return __builtin_fma(-k, m.q, x);
+}
+
+// |x| < 2^50 -> canonical [0, q) as uint32
+__device__ __forceinline__ uint32_t canon_f(double x, const ModF &m) {
+  double r = reduce_f(x, m);
+  r = r < 0.0 ? r + m.q : r;
+  r = r >= m.q ? r - m.q : r;
+  return (uint32_t)r;
+}
+
+// |x| < 2^50 -> centred [-(q-1)/2, (q-1)/2] as int32
+__device__ __forceinline__ int32_t centre_f(double x, const ModF &m) {
+  double r = reduce_f(x, m);
+  double h = (m.q - 1.0) * 0.5;
+  r = r > h ? r - m.q : r;
+  r = r < -h ? r + m.q : r;
+  return (int32_t)r;
+}
+
+}  // namespace ck

@@ -1,0 +1,66 @@
+// Element-wise launch descriptors (see ck_elem.hip).
+#pragma once
+
+#include "ck_internal.h"
+
+namespace ck {
+
+constexpr int kMaxPolys = 3;
+constexpr int kMaxDnum = 16;
+
+// Limb -> prime map of an NP (num_main, num_ter, num_aux): limb y < num_q
+// uses global prime q_prime0 + y, aux limb y uses aux_prime0 + (y - num_q).
+struct EwNp {
+  int num_q;
+  int num_total;
+  int q_prime0;
+  int aux_prime0;
+};
+
+// Up to kMaxPolys polynomials sharing one layout; aux limbs sit `gap` limbs
+// after the q part (the reference's DvView extra_, PtrList.h:37-63).
+struct EwOperand {
+  uint32_t *ptr[kMaxPolys];
+  int gap;
+};
+
+struct KeyAccumArgs {
+  const uint32_t *modup;  // beta blocks of num_total limbs
+  const uint32_t *key_b[kMaxDnum];  // full-size key limbs, global prime index
+  const uint32_t *key_a[kMaxDnum];
+  int group[kMaxDnum];
+  int ngroups;
+  const uint32_t *add_b;  // num_q limbs or null
+  const uint32_t *add_a;
+  const uint32_t *p_prod;  // by global prime
+  uint32_t *out_b;
+  uint32_t *out_a;
+};
+
+void launch_binary(const Context &ctx, int op, const EwNp &np,
+                   const EwOperand &x, const EwOperand &y, const EwOperand &out,
+                   int npoly, hipStream_t st);
+void launch_unary(const Context &ctx, int op, const EwNp &np,
+                  const EwOperand &x, const EwOperand &out, int npoly,
+                  hipStream_t st);
+void launch_const(const Context &ctx, int op, const EwNp &np,
+                  const EwOperand &x, const uint32_t *consts, int const_gap,
+                  const EwOperand &out, int npoly, hipStream_t st);
+void launch_tensor(const Context &ctx, const EwNp &np, const EwOperand &c1,
+                   const EwOperand &c2, const EwOperand &out, hipStream_t st);
+void launch_key_accum(const Context &ctx, const EwNp &np,
+                      const KeyAccumArgs &A, hipStream_t st);
+void launch_permute(const Context &ctx, const EwNp &np, const EwOperand &x,
+                    const EwOperand &out, uint32_t g, int npoly,
+                    hipStream_t st);
+
+inline EwNp make_np(const Context &c, int num_main, int num_ter, int num_aux) {
+  EwNp np;
+  np.num_q = num_main + num_ter;
+  np.num_total = np.num_q + num_aux;
+  np.q_prime0 = c.max_ter - num_ter;
+  np.aux_prime0 = c.L;
+  return np;
+}
+
+}  // namespace ck

@@ -1,0 +1,263 @@
+// Element-wise kernels over RNS polynomials (reference:
+// src/core/ElementWise.cu). All are HBM-bound: each thread moves 4
+// consecutive coefficients of one limb with 16-byte loads/stores, the limb
+// (and so the prime) is uniform per workgroup, and products are Montgomery
+// products of stored values exactly as the reference's MultMontgomery
+// (include/common/Basic.cuh:251-260), so outputs are bit-identical.
+#include "ck_elem.h"
+
+namespace ck {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kVec = 4;
+constexpr int kBlocksPerLimb = 65536 / (kThreads * kVec);
+
+__device__ __forceinline__ int ew_prime(const EwNp &np, int y) {
+  return y < np.num_q ? np.q_prime0 + y : np.aux_prime0 + (y - np.num_q);
+}
+__device__ __forceinline__ size_t ew_off(const EwOperand &o, const EwNp &np,
+                                         int y, int k) {
+  return (size_t)(y < np.num_q ? y : y + o.gap) * 65536 + k;
+}
+
+#define EW_PROLOGUE                                    \
+  const int y = blockIdx.y;                            \
+  const int k = (blockIdx.x * kThreads + threadIdx.x) * kVec; \
+  const int p = ew_prime(np, y);                       \
+  const uint32_t q = P.q[p];                           \
+  const int32_t qinv = P.qinv[p];                      \
+  (void)qinv;
+
+__device__ __forceinline__ uint4 ld4(const uint32_t *p) {
+  return *reinterpret_cast<const uint4 *>(p);
+}
+__device__ __forceinline__ void st4(uint32_t *p, uint4 v) {
+  *reinterpret_cast<uint4 *>(p) = v;
+}
+
+template <typename F>
+__device__ __forceinline__ uint4 map2(uint4 a, uint4 b, F f) {
+  return make_uint4(f(a.x, b.x), f(a.y, b.y), f(a.z, b.z), f(a.w, b.w));
+}
+
+// op: 0 add, 1 sub, 2 mult (Montgomery)
+template <int OP>
+__global__ __launch_bounds__(kThreads) void binary_kernel(
+    const EwNp np, const PrimeTab P, EwOperand x, EwOperand yv, EwOperand out,
+    int npoly) {
+  EW_PROLOGUE
+  for (int j = 0; j < npoly; j++) {
+    uint4 a = ld4(x.ptr[j] + ew_off(x, np, y, k));
+    uint4 b = ld4(yv.ptr[j] + ew_off(yv, np, y, k));
+    uint4 r;
+    if (OP == 0)
+      r = map2(a, b, [&](uint32_t u, uint32_t v) { return add_mod(u, v, q); });
+    else if (OP == 1)
+      r = map2(a, b, [&](uint32_t u, uint32_t v) { return sub_mod(u, v, q); });
+    else
+      r = map2(a, b,
+               [&](uint32_t u, uint32_t v) { return mont_mul(u, v, q, qinv); });
+    st4(out.ptr[j] + ew_off(out, np, y, k), r);
+  }
+}
+
+// op: 0 neg, 1 copy
+template <int OP>
+__global__ __launch_bounds__(kThreads) void unary_kernel(const EwNp np,
+                                                         const PrimeTab P,
+                                                         EwOperand x,
+                                                         EwOperand out,
+                                                         int npoly) {
+  EW_PROLOGUE
+  for (int j = 0; j < npoly; j++) {
+    uint4 a = ld4(x.ptr[j] + ew_off(x, np, y, k));
+    uint4 r = a;
+    if (OP == 0)
+      r = map2(a, a, [&](uint32_t u, uint32_t) { return u ? q - u : 0u; });
+    st4(out.ptr[j] + ew_off(out, np, y, k), r);
+  }
+}
+
+// CPAccum<const> with one term = MultConst: x * c_y (c in Montgomery form)
+// op: 0 mult, 1 add (c + x), 2 sub (x - c)
+template <int OP>
+__global__ __launch_bounds__(kThreads) void const_kernel(
+    const EwNp np, const PrimeTab P, EwOperand x, const uint32_t *consts,
+    int const_gap, EwOperand out, int npoly) {
+  EW_PROLOGUE
+  const uint32_t c = consts[y < np.num_q ? y : y + const_gap];
+  for (int j = 0; j < npoly; j++) {
+    uint4 a = ld4(x.ptr[j] + ew_off(x, np, y, k));
+    uint4 r;
+    if (OP == 0)
+      r = map2(a, a, [&](uint32_t u, uint32_t) { return mont_mul(u, c, q, qinv); });
+    else if (OP == 1)
+      r = map2(a, a, [&](uint32_t u, uint32_t) { return add_mod(u, c, q); });
+    else
+      r = map2(a, a, [&](uint32_t u, uint32_t) { return sub_mod(u, c, q); });
+    st4(out.ptr[j] + ew_off(out, np, y, k), r);
+  }
+}
+
+// Tensor (ElementWise.cu:504-546): (b1 b2, (b1+a1)(b2+a2) - b1 b2 - a1 a2,
+// a1 a2) with the Karatsuba middle term on signed lazy sums.
+__global__ __launch_bounds__(kThreads) void tensor_kernel(
+    const EwNp np, const PrimeTab P, EwOperand c1, EwOperand c2,
+    EwOperand out) {
+  EW_PROLOGUE
+  uint4 b1 = ld4(c1.ptr[0] + ew_off(c1, np, y, k));
+  uint4 a1 = ld4(c1.ptr[1] + ew_off(c1, np, y, k));
+  uint4 b2 = ld4(c2.ptr[0] + ew_off(c2, np, y, k));
+  uint4 a2 = ld4(c2.ptr[1] + ew_off(c2, np, y, k));
+  uint32_t bb[4], aa[4], mid[4];
+  const uint32_t B1[4] = {b1.x, b1.y, b1.z, b1.w};
+  const uint32_t A1[4] = {a1.x, a1.y, a1.z, a1.w};
+  const uint32_t B2[4] = {b2.x, b2.y, b2.z, b2.w};
+  const uint32_t A2[4] = {a2.x, a2.y, a2.z, a2.w};
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    bb[i] = mont_mul(B1[i], B2[i], q, qinv);
+    aa[i] = mont_mul(A1[i], A2[i], q, qinv);
+    const int32_t s1 = (int32_t)(B1[i] - q) + (int32_t)A1[i];  // [-q, q-2]
+    const int32_t s2 = (int32_t)(B2[i] - q) + (int32_t)A2[i];
+    uint32_t m = mont_reduce((int64_t)s1 * (int64_t)s2, q, qinv);
+    m = sub_mod(m, bb[i], q);
+    mid[i] = sub_mod(m, aa[i], q);
+  }
+  const size_t o = ew_off(out, np, y, k);
+  st4(out.ptr[0] + o, make_uint4(bb[0], bb[1], bb[2], bb[3]));
+  st4(out.ptr[1] + o, make_uint4(mid[0], mid[1], mid[2], mid[3]));
+  st4(out.ptr[2] + o, make_uint4(aa[0], aa[1], aa[2], aa[3]));
+}
+
+// Key inner product of MultKeyNoModDown (Context.cpp:609-712 ->
+// ElementWise::PAccum, CPAccum<2,false>) fused with the P * (b, a) fold
+// (CAccum, CPAccumAdd<2,true>):
+//   acc_x[y] = sum_g modup_g[y] (*) key_x_g[prime(y)]  (+ P (*) add_x[y])
+__global__ __launch_bounds__(kThreads) void key_accum_kernel(
+    const EwNp np, const PrimeTab P, const KeyAccumArgs A) {
+  EW_PROLOGUE
+  const size_t tot_stride = (size_t)np.num_total * 65536;
+  uint32_t rb[4] = {0, 0, 0, 0}, ra[4] = {0, 0, 0, 0};
+  const size_t my = (size_t)y * 65536 + k;
+  const size_t kp = (size_t)p * 65536 + k;
+  for (int gi = 0; gi < A.ngroups; gi++) {
+    const int g = A.group[gi];
+    uint4 mu = ld4(A.modup + (size_t)g * tot_stride + my);
+    uint4 kb = ld4(A.key_b[g] + kp);
+    uint4 ka = ld4(A.key_a[g] + kp);
+    const uint32_t M[4] = {mu.x, mu.y, mu.z, mu.w};
+    const uint32_t KB[4] = {kb.x, kb.y, kb.z, kb.w};
+    const uint32_t KA[4] = {ka.x, ka.y, ka.z, ka.w};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      rb[i] = add_mod(rb[i], mont_mul(M[i], KB[i], q, qinv), q);
+      ra[i] = add_mod(ra[i], mont_mul(M[i], KA[i], q, qinv), q);
+    }
+  }
+  if (y < np.num_q) {
+    const uint32_t pp = A.p_prod[p];
+    if (A.add_b) {
+      uint4 v = ld4(A.add_b + my);
+      const uint32_t V[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        rb[i] = add_mod(rb[i], mont_mul(pp, V[i], q, qinv), q);
+    }
+    if (A.add_a) {
+      uint4 v = ld4(A.add_a + my);
+      const uint32_t V[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        ra[i] = add_mod(ra[i], mont_mul(pp, V[i], q, qinv), q);
+    }
+  }
+  st4(A.out_b + my, make_uint4(rb[0], rb[1], rb[2], rb[3]));
+  st4(A.out_a + my, make_uint4(ra[0], ra[1], ra[2], ra[3]));
+}
+
+// Permute (ElementWise.cu:370-391): dst[i] = src[brv17((brv17(i)+1) g - 1)]
+// (log N + 1 = 17 bits; wraps mod 2N). Gathers are L2-served; writes are
+// coalesced.
+__global__ __launch_bounds__(kThreads) void permute_kernel(
+    const EwNp np, EwOperand x, EwOperand out, uint32_t g, int npoly) {
+  const int y = blockIdx.y;
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t e = (__brev((uint32_t)k) >> 15) + 1u;
+  const uint32_t s = __brev((e * g - 1u) & 0x1FFFFu) >> 15;
+  for (int j = 0; j < npoly; j++) {
+    const uint32_t *src = x.ptr[j] + (size_t)(y < np.num_q ? y : y + x.gap) * 65536;
+    out.ptr[j][(size_t)(y < np.num_q ? y : y + out.gap) * 65536 + k] = src[s];
+  }
+}
+
+}  // namespace
+
+static dim3 ew_grid(const EwNp &np) { return dim3(kBlocksPerLimb, np.num_total); }
+
+void launch_binary(const Context &ctx, int op, const EwNp &np,
+                   const EwOperand &x, const EwOperand &y, const EwOperand &out,
+                   int npoly, hipStream_t st) {
+  if (np.num_total <= 0) return;
+  if (op == 0)
+    hipLaunchKernelGGL(binary_kernel<0>, ew_grid(np), dim3(kThreads), 0, st, np,
+                       ctx.ptab, x, y, out, npoly);
+  else if (op == 1)
+    hipLaunchKernelGGL(binary_kernel<1>, ew_grid(np), dim3(kThreads), 0, st, np,
+                       ctx.ptab, x, y, out, npoly);
+  else
+    hipLaunchKernelGGL(binary_kernel<2>, ew_grid(np), dim3(kThreads), 0, st, np,
+                       ctx.ptab, x, y, out, npoly);
+}
+
+void launch_unary(const Context &ctx, int op, const EwNp &np,
+                  const EwOperand &x, const EwOperand &out, int npoly,
+                  hipStream_t st) {
+  if (np.num_total <= 0) return;
+  if (op == 0)
+    hipLaunchKernelGGL(unary_kernel<0>, ew_grid(np), dim3(kThreads), 0, st, np,
+                       ctx.ptab, x, out, npoly);
+  else
+    hipLaunchKernelGGL(unary_kernel<1>, ew_grid(np), dim3(kThreads), 0, st, np,
+                       ctx.ptab, x, out, npoly);
+}
+
+void launch_const(const Context &ctx, int op, const EwNp &np,
+                  const EwOperand &x, const uint32_t *consts, int const_gap,
+                  const EwOperand &out, int npoly, hipStream_t st) {
+  if (np.num_total <= 0) return;
+  if (op == 0)
+    hipLaunchKernelGGL(const_kernel<0>, ew_grid(np), dim3(kThreads), 0, st, np,
+                       ctx.ptab, x, consts, const_gap, out, npoly);
+  else if (op == 1)
+    hipLaunchKernelGGL(const_kernel<1>, ew_grid(np), dim3(kThreads), 0, st, np,
+                       ctx.ptab, x, consts, const_gap, out, npoly);
+  else
+    hipLaunchKernelGGL(const_kernel<2>, ew_grid(np), dim3(kThreads), 0, st, np,
+                       ctx.ptab, x, consts, const_gap, out, npoly);
+}
+
+void launch_tensor(const Context &ctx, const EwNp &np, const EwOperand &c1,
+                   const EwOperand &c2, const EwOperand &out, hipStream_t st) {
+  if (np.num_total <= 0) return;
+  hipLaunchKernelGGL(tensor_kernel, ew_grid(np), dim3(kThreads), 0, st, np,
+                     ctx.ptab, c1, c2, out);
+}
+
+void launch_key_accum(const Context &ctx, const EwNp &np,
+                      const KeyAccumArgs &A, hipStream_t st) {
+  if (np.num_total <= 0) return;
+  hipLaunchKernelGGL(key_accum_kernel, ew_grid(np), dim3(kThreads), 0, st, np,
+                     ctx.ptab, A);
+}
+
+void launch_permute(const Context &ctx, const EwNp &np, const EwOperand &x,
+                    const EwOperand &out, uint32_t g, int npoly,
+                    hipStream_t st) {
+  if (np.num_total <= 0) return;
+  hipLaunchKernelGGL(permute_kernel, dim3(65536 / kThreads, np.num_total),
+                     dim3(kThreads), 0, st, np, x, out, g, npoly);
+}
+
+}  // namespace ck

@@ -1,0 +1,160 @@
+// Internal declarations of the MI355X CKKS key-switching core.
+//
+// Not part of the C ABI (include/cheddar_amd.h); shared by the HIP kernels
+// (ck_*.hip) and the host orchestration (ck_context.cpp, ck_ops.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "ck_arith.h"
+
+namespace ck {
+
+// ---- status codes (mirrored in include/cheddar_amd.h) ----
+enum Status : int {
+  kOk = 0,
+  kErrInvalidArg = 1,
+  kErrUnsupported = 2,
+  kErrHip = 3,
+  kErrOutOfMemory = 4,
+};
+
+// Device view of the per-prime constants (global prime order
+// [terminal reversed | main | auxiliary]).
+struct PrimeTab {
+  const uint32_t *q;
+  const int32_t *qinv;  // q^-1 mod 2^32 (signed)
+  const double *qd;     // q as double
+  const double *qr;     // 1/q
+};
+
+// Per-launch limb plan, passed by value as a kernel argument so that no
+// host-to-device copy happens inside an operation (graph-capture safe).
+constexpr int kMaxLaunchLimbs = 112;
+
+enum NttFlags : int {
+  kNttPreMul = 1,           // forward: x *= c1 before the transform
+  kNttModDownEpilogue = 2,  // forward: out = (src2 * c2 - NTT(x)) * c3
+  kNttEpiloguePad = 4,      // forward: apply c2 in the epilogue
+  kInttCentred = 8,         // inverse: centred int32 output
+};
+
+struct NttLaunch {
+  // up to four base pointers per role; limb i uses src[sel & 3],
+  // dst[(sel >> 2) & 3], src2[(sel >> 4) & 3]
+  const uint32_t *src[4];
+  uint32_t *dst[4];
+  const uint32_t *src2[4];
+  int n;
+  int flags;
+  uint8_t prime[kMaxLaunchLimbs];
+  uint8_t sel[kMaxLaunchLimbs];
+  uint16_t src_limb[kMaxLaunchLimbs];
+  uint16_t dst_limb[kMaxLaunchLimbs];
+  int16_t src2_limb[kMaxLaunchLimbs];
+  uint32_t c1[kMaxLaunchLimbs];  // plain-form constants (already carry R^-1)
+  uint32_t c2[kMaxLaunchLimbs];
+  uint32_t c3[kMaxLaunchLimbs];
+};
+
+// Fast base conversion launch: dst limb m (global prime dst_prime[m], at
+// dst_limb[m] of dst) = ReduceMontgomery(sum_i src[i] * table[m * s + i]).
+constexpr int kMaxBconvDst = 112;
+constexpr int kMaxBconvSrc = 16;
+struct BconvLaunch {
+  const int32_t *src[4];  // per poly (grid.z): s contiguous limbs, centred
+  uint32_t *dst[4];       // per poly
+  const int32_t *table;   // d x s, device
+  int s, d;
+  uint8_t dst_prime[kMaxBconvDst];
+  uint16_t dst_limb[kMaxBconvDst];
+};
+
+// Tables of one ModDown flavour (ModDown, Rescale, ModDownAndRescale) at one
+// level, mirroring ModSwitchHandler's constants (ModSwitch.cu:233-411).
+struct ModDownPlan {
+  bool valid = false;
+  int src_len = 0, dst_len = 0;
+  std::vector<int> src_limb;   // limb index in the source poly
+  std::vector<int> src_prime;  // global prime index
+  std::vector<uint32_t> c1;    // INTT post-constant (plain form)
+  std::vector<int> dst_prime;  // global prime index of each output limb
+  std::vector<int> src2_limb;  // source limb added in the epilogue, -1 none
+  std::vector<uint32_t> pad;   // epilogue padding (plain)
+  std::vector<uint32_t> inv_prod;  // epilogue (prod src)^-1 (plain)
+  bool use_pad = false;
+  int32_t *d_table = nullptr;  // dst_len x src_len
+};
+
+struct ModUpGroup {
+  int s0 = 0, s1 = 0;           // own limb range [s0, s1) within the level
+  std::vector<int> dst_limb;    // limb index in the (num_q + alpha) output
+  std::vector<int> dst_prime;   // global prime
+  int32_t *d_table = nullptr;   // dst x src
+};
+
+struct LevelPlan {
+  int nm = 0, nt = 0, num_q = 0, beta = 0, pad = 0;
+  std::vector<int> q_prime;        // global prime of each q limb
+  std::vector<uint32_t> modup_c1;  // per q limb (plain form)
+  std::vector<ModUpGroup> groups;  // beta entries (empty ones have s1 <= s0)
+  ModDownPlan moddown, rescale, moddown_rescale;
+};
+
+struct Context {
+  int log_n = 0, n = 0;
+  int max_main = 0, max_ter = 0, alpha = 0, L = 0, dnum = 0, max_level = 0;
+  int num_primes = 0;
+  std::vector<int> lv_main, lv_ter;
+  std::vector<uint32_t> primes;
+  std::vector<int32_t> qinv;
+  std::vector<uint32_t> p_prod_mont;  // P mod q_i * R (q primes)
+  std::vector<uint32_t> galois;       // 5^i mod 2N, i in [0, N/2]
+
+  // device tables
+  uint32_t *d_q = nullptr;
+  int32_t *d_qinv = nullptr;
+  double *d_qd = nullptr, *d_qr = nullptr;
+  uint32_t *d_tw_fwd = nullptr;  // psi^brv(j), plain, per prime
+  uint32_t *d_tw_inv = nullptr;  // psi^-brv(j), plain, per prime
+  uint32_t *d_p_prod = nullptr;  // P mod q * R, indexed by global q prime
+  PrimeTab ptab{};
+
+  std::vector<LevelPlan> levels;
+
+  // workspace for multi-kernel operations (one operation at a time per
+  // context; concurrent calls on different streams race)
+  uint32_t *ws = nullptr;
+  size_t ws_words = 0;
+
+  std::vector<void *> allocations;
+};
+
+// ---- context.cpp ----
+int context_create(int log_n, const uint32_t *main_primes, int n_main,
+                   const uint32_t *ter_primes, int n_ter,
+                   const uint32_t *aux_primes, int n_aux,
+                   const int *level_cfg, int n_levels, Context **out);
+void context_destroy(Context *ctx);
+void set_error(const std::string &msg);
+const char *last_error();
+int check_hip(hipError_t e, const char *what);
+int global_prime(const Context &c, int num_main, int num_ter, int limb);
+
+// host number theory (independent restatement of PrimeUtils.h)
+uint32_t h_mulmod(uint32_t a, uint32_t b, uint32_t q);
+uint32_t h_powmod(uint32_t a, uint64_t e, uint32_t q);
+uint32_t h_invmod(uint32_t a, uint32_t q);
+uint32_t h_to_mont(uint32_t a, uint32_t q);
+
+// ---- launchers ----
+void launch_ntt_fwd(const Context &ctx, const NttLaunch &L, hipStream_t st);
+void launch_intt(const Context &ctx, const NttLaunch &L, hipStream_t st);
+void launch_bconv(const Context &ctx, const BconvLaunch &B, int npoly,
+                  hipStream_t st);
+
+}  // namespace ck

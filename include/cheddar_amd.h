@@ -1,0 +1,212 @@
+/*
+ * cheddar_amd.h -- C ABI of the MI355X (gfx950) CKKS key-switching core.
+ *
+ * Drop-in boundary for the handler layer of scale-snu/cheddar-fhe (32-bit
+ * word path). Each entry point names the reference interface it replaces.
+ *
+ * Conventions (identical to the reference):
+ *   - a polynomial is limb-major u32 [limb][N], N = 2^16, residues in [0,q);
+ *   - limb order of an NP (num_main, num_ter, num_aux) is
+ *     [terminal primes reversed | main primes | auxiliary primes]
+ *     (src/core/Parameter.cu:103-111);
+ *   - NTT-domain data hold x * 2^32 mod q (Montgomery form); products of
+ *     stored values are Montgomery products x * y * 2^-32;
+ *   - evaluation keys are full size (EvaluationKey, include/core/Container.h:
+ *     288-345): dnum blocks, each a (L + alpha)-limb polynomial in global
+ *     prime order, passed as host arrays of dnum device pointers;
+ *   - `src_gap` is the reference's DvView extra_ (include/common/PtrList.h:
+ *     37-63) in limbs: aux limb a of a source sits at limb num_q + a + gap.
+ *
+ * Every pointer to polynomial data is a device pointer; `stream` is a
+ * hipStream_t (NULL = default stream). Entry points enqueue work and return
+ * without synchronising. Operations of one context share one workspace:
+ * calls on one context must not overlap in time (serialise them on one
+ * stream). Errors return a non-zero status and set a message readable with
+ * ck_last_error(); nothing exits the process (the reference's AssertTrue
+ * calls std::exit, include/common/Assert.h:9-21).
+ */
+#ifndef CHEDDAR_AMD_H_
+#define CHEDDAR_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CK_OK 0
+#define CK_ERR_INVALID_ARG 1
+#define CK_ERR_UNSUPPORTED 2
+#define CK_ERR_HIP 3
+#define CK_ERR_OUT_OF_MEMORY 4
+
+/* ModDown flavours (ModSwitchHandler::ModDown / Rescale / ModDownAndRescale,
+ * include/core/ModSwitch.h:38-44) */
+#define CK_MODDOWN 0
+#define CK_RESCALE 1
+#define CK_MODDOWN_RESCALE 2
+
+typedef struct ck_context ck_context;
+
+/* Message of the last failing call on this thread. */
+const char *ck_last_error(void);
+/* ABI version (major * 100 + minor). */
+int ck_version(void);
+
+/* ---- Parameter + Context -------------------------------------------- */
+
+/* Replaces Parameter<word>::Parameter (include/core/Parameter.h:42-47,
+ * src/core/Parameter.cu:20-193) together with Context<word>::Create
+ * (include/core/Context.h:70, src/core/Context.cpp:62-124): validates the
+ * primes and the level table, builds twiddles, ModUp/ModDown constants and
+ * the P table, and allocates the workspace. level_config holds n_levels
+ * pairs (num_main, num_ter), as the parameter JSON's "level_config". */
+int ck_context_create(int log_degree, const uint32_t *main_primes, int n_main,
+                      const uint32_t *ter_primes, int n_ter,
+                      const uint32_t *aux_primes, int n_aux,
+                      const int *level_config, int n_levels,
+                      ck_context **out);
+int ck_context_destroy(ck_context *ctx);
+/* out[0..9] = log_n, N, max_main, max_ter, alpha, L, dnum, max_level,
+ * num_primes, workspace bytes / 2^20 */
+int ck_context_info(const ck_context *ctx, int *out);
+/* Parameter::LevelToNP (src/core/Parameter.cu:287-296) */
+int ck_level_to_np(const ck_context *ctx, int level, int *num_main,
+                   int *num_ter);
+/* Parameter::NPToLevel (src/core/Parameter.cu:298-315); -1 when absent */
+int ck_np_to_level(const ck_context *ctx, int num_main, int num_ter);
+/* beta of ModSwitchHandler at a level (src/core/ModSwitch.cu:181-185) */
+int ck_modup_beta(const ck_context *ctx, int level);
+/* Parameter::GetGaloisFactor (src/core/Parameter.cu:195-200) and
+ * ElementWiseHandler::PermuteAmountToGaloisFactor (ElementWise.cu:955-964):
+ * rot in [0, N/2] -> 5^rot mod 2N, rot == -1 -> 2N - 1 (conjugation) */
+int ck_galois_factor(const ck_context *ctx, int rot, uint32_t *out);
+/* the prime of global index pidx ([ter reversed | main | aux]) */
+int ck_prime(const ck_context *ctx, int pidx, uint32_t *out);
+
+/* ---- NTTHandler (include/core/NTT.h:56-81) --------------------------- */
+
+/* NTTHandler::NTT (src/core/NTT.cu:606-671): forward negacyclic NTT of every
+ * limb of NP (num_main, num_ter, num_aux); montgomery_conversion multiplies
+ * the input by 2^32 first. dst may equal src. */
+int ck_ntt(ck_context *ctx, uint32_t *dst, const uint32_t *src, int num_main,
+           int num_ter, int num_aux, int src_gap, int montgomery_conversion,
+           void *stream);
+/* NTTHandler::INTT (src/core/NTT.cu:673-741): inverse NTT times N^-1;
+ * montgomery_conversion = 1 leaves the Montgomery form. dst may equal src. */
+int ck_intt(ck_context *ctx, uint32_t *dst, const uint32_t *src, int num_main,
+            int num_ter, int num_aux, int src_gap, int montgomery_conversion,
+            void *stream);
+/* NTTHandler::INTTAndMultConst (src/core/NTT.cu:743-813): dst = INTT(src)
+ * (*) c per limb (Montgomery product, no N^-1), canonical or centred
+ * (normalize: int32 in [-(q-1)/2, (q-1)/2]). consts is a HOST array of one
+ * value per limb (the reference passes a device view; host constants keep
+ * the call free of device-to-host reads). */
+int ck_intt_mult_const(ck_context *ctx, uint32_t *dst, const uint32_t *src,
+                       int num_main, int num_ter, int num_aux, int src_gap,
+                       const uint32_t *consts, int normalize, void *stream);
+
+/* ---- ModSwitchHandler (include/core/ModSwitch.h:38-44) ---------------- */
+
+/* ModSwitchHandler::ModUp (src/core/ModSwitch.cu:429-509) at `level`: src
+ * holds the level's num_q NTT-domain limbs; dst holds beta blocks of
+ * (num_q + alpha) limbs, block g = group g's decomposition raised to Q*P.
+ * Blocks of groups lying wholly in the terminal padding are not written. */
+int ck_modup(ck_context *ctx, int level, uint32_t *dst, const uint32_t *src,
+             void *stream);
+/* ModSwitchHandler::ModDown / Rescale / ModDownAndRescale
+ * (src/core/ModSwitch.cu:511-684) of npoly (1..4) polynomials at `level`.
+ * kind = CK_MODDOWN: src (num_q + alpha) limbs -> dst num_q limbs;
+ * CK_RESCALE: src num_q -> dst num_q(level - 1);
+ * CK_MODDOWN_RESCALE: src (num_q + alpha) -> dst num_q(level - 1).
+ * dst and src are host arrays of npoly device pointers; dst != src. */
+int ck_moddown(ck_context *ctx, int level, int kind, int npoly,
+               uint32_t *const *dst, const uint32_t *const *src,
+               void *stream);
+
+/* ---- ElementWiseHandler (include/core/ElementWise.h:52-116) ---------- */
+/* npoly polynomials of NP (num_main, num_ter, num_aux) with no limb gap;
+ * dst, x, y are host arrays of npoly (1..3) device pointers. */
+
+/* ElementWiseHandler::Add (ElementWise.cu:688-716, kernel Sum :32-71) */
+int ck_add(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly,
+           uint32_t *const *dst, const uint32_t *const *x,
+           const uint32_t *const *y, void *stream);
+/* ElementWiseHandler::Sub (kernel Sub, ElementWise.cu:74-98) */
+int ck_sub(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly,
+           uint32_t *const *dst, const uint32_t *const *x,
+           const uint32_t *const *y, void *stream);
+/* ElementWiseHandler::Mult (kernel Mult, ElementWise.cu:120-143):
+ * Montgomery product */
+int ck_mult(ck_context *ctx, int num_main, int num_ter, int num_aux,
+            int npoly, uint32_t *const *dst, const uint32_t *const *x,
+            const uint32_t *const *y, void *stream);
+/* ElementWiseHandler::Neg (kernel Neg, ElementWise.cu:100-118) */
+int ck_neg(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly,
+           uint32_t *const *dst, const uint32_t *const *x, void *stream);
+/* ElementWiseHandler::MultConst (CPAccum<1,const>, ElementWise.cu:225-287):
+ * x (*) c[limb]; consts is a DEVICE array of num_total values */
+int ck_mult_const(ck_context *ctx, int num_main, int num_ter, int num_aux,
+                  int npoly, uint32_t *const *dst, const uint32_t *const *x,
+                  const uint32_t *consts, void *stream);
+/* ElementWiseHandler::Tensor (ElementWise.cu:921-953, kernel :504-546):
+ * (d0, d1, d2) = (b1 b2, b1 a2 + a1 b2, a1 a2) on num_q limbs */
+int ck_tensor(ck_context *ctx, int num_main, int num_ter, uint32_t *d0,
+              uint32_t *d1, uint32_t *d2, const uint32_t *b1,
+              const uint32_t *a1, const uint32_t *b2, const uint32_t *a2,
+              void *stream);
+/* ElementWiseHandler::Permute (ElementWise.cu:966-1002, kernel :370-391):
+ * NTT-domain automorphism by 5^rot (rot >= 0) or conjugation (rot == -1);
+ * dst != x. */
+int ck_permute(ck_context *ctx, int num_main, int num_ter, int num_aux,
+               int npoly, uint32_t *const *dst, const uint32_t *const *x,
+               int rot, void *stream);
+
+/* ---- Context evaluator (include/core/Context.h:101-423) --------------- */
+/* Ciphertext polynomials are num_q(level) contiguous limbs; keys are
+ * host arrays of dnum device pointers to full-size key blocks. */
+
+/* Context::MultKeyNoModDown (src/core/Context.cpp:609-712): accum (num_q +
+ * alpha limbs each) = sum_g ModUp(x)_g (*) key_g, plus P (*) add on the q
+ * limbs (add_b / add_a may be NULL). */
+int ck_mult_key_no_moddown(ck_context *ctx, int level, uint32_t *accum_b,
+                           uint32_t *accum_a, const uint32_t *x,
+                           const uint32_t *add_b, const uint32_t *add_a,
+                           const uint32_t *const *key_b,
+                           const uint32_t *const *key_a, void *stream);
+/* Context::MultKey (src/core/Context.cpp:584-607) of a ciphertext (b, a) */
+int ck_mult_key(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
+                const uint32_t *b, const uint32_t *a,
+                const uint32_t *const *key_b, const uint32_t *const *key_a,
+                void *stream);
+/* Context::Relinearize / RelinearizeRescale (src/core/Context.cpp:561-565,
+ * :714-737) of a 3-poly ciphertext (d0, d1, d2) */
+int ck_relinearize(ck_context *ctx, int level, uint32_t *out_b,
+                   uint32_t *out_a, const uint32_t *d0, const uint32_t *d1,
+                   const uint32_t *d2, const uint32_t *const *key_b,
+                   const uint32_t *const *key_a, int rescale, void *stream);
+/* Context::HMult (src/core/Context.cpp:862-871): Tensor + Relinearize, with
+ * the merged ModDownAndRescale when rescale != 0 (output num_q(level - 1)
+ * limbs per poly, else num_q(level)). */
+int ck_hmult(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
+             const uint32_t *b1, const uint32_t *a1, const uint32_t *b2,
+             const uint32_t *a2, const uint32_t *const *key_b,
+             const uint32_t *const *key_a, int rescale, void *stream);
+/* Context::HRot (src/core/Context.cpp:776-791): MultKey then Permute by
+ * 5^rot; rot is reduced mod N/2 (num_slots), rot == 0 copies. */
+int ck_hrot(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
+            const uint32_t *b, const uint32_t *a, const uint32_t *const *key_b,
+            const uint32_t *const *key_a, int rot, void *stream);
+/* Context::HConj (src/core/Context.cpp:793-798) */
+int ck_hconj(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
+             const uint32_t *b, const uint32_t *a, const uint32_t *const *key_b,
+             const uint32_t *const *key_a, void *stream);
+/* Context::Rescale (src/core/Context.cpp:739-774) of npoly (2 or 3)
+ * polynomials at `level` -> level - 1 */
+int ck_rescale(ck_context *ctx, int level, int npoly, uint32_t *const *dst,
+               const uint32_t *const *src, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHEDDAR_AMD_H_ */

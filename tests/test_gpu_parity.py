@@ -1,0 +1,312 @@
+"""GPU parity: every entry point of the HIP library (through its C ABI)
+against the CPU oracle on the same seeded inputs, bit for bit, at the full
+ring dimension N = 2^16.
+
+Parameter sets: the 24 + 8 prime chain of the plain key-switching workload
+(q_i = 1 mod 2^17, largest primes below 2^31) and the reference's
+bootparam_40.json (terminal primes, 43 main, 12 auxiliary).
+"""
+import numpy as np
+import pytest
+
+from conftest import SEED_BASE
+from gpu_util import assert_same, dev_key, full_key, uniform
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def ck():
+    import cheddar_amd
+    return cheddar_amd
+
+
+@pytest.fixture(scope="module")
+def ctx24(ck, rns24):
+    return ck.Context.from_json(rns24)
+
+
+@pytest.fixture(scope="module")
+def ctx40(ck, boot40):
+    return ck.Context.from_json(boot40)
+
+
+def sync():
+    torch.cuda.synchronize()
+
+
+# ---------------------------------------------------------------------------
+# NTTHandler
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("mont_conv", [False, True])
+def test_ntt_forward_all_primes(ck, ctx24, oracle_rns24, mont_conv):
+    P = oracle_rns24
+    x = uniform(P, P.np_primes(24, 0, 8), SEED_BASE + 1)
+    want = P.ntt_np(x, 24, 0, 8, mont_conv=mont_conv)
+    d = ck.to_device(x)
+    out = torch.empty_like(d)
+    ctx24.ntt(out, d, 24, 0, 8, mont_conv=mont_conv)
+    sync()
+    assert_same(ck.to_host(out), want, "ntt")
+
+
+def test_ntt_in_place_and_extremes(ck, ctx24, oracle_rns24):
+    P = oracle_rns24
+    pidx = P.np_primes(24, 0, 0)
+    q = np.array([P.prime(i) for i in pidx], dtype=np.uint64)[:, None]
+    for x in (np.zeros((24, P.n), np.uint32), (q - 1).astype(np.uint32) + np.zeros((24, P.n), np.uint32)):
+        want = P.ntt_np(x, 24, 0, 0)
+        d = ck.to_device(x)
+        ctx24.ntt(d, d, 24, 0, 0)
+        sync()
+        assert_same(ck.to_host(d), want, "ntt in place")
+
+
+def test_ntt_level_slice_with_gap(ck, ctx24, oracle_rns24):
+    """NP (7 main, 0 ter, 8 aux) read from a buffer whose aux limbs sit
+    after a 17-limb gap (the reference's extra_ view of a max-level key)."""
+    P = oracle_rns24
+    full = uniform(P, P.np_primes(24, 0, 8), SEED_BASE + 2)
+    sub = np.concatenate([full[:7], full[24:]])
+    want = P.ntt_np(sub, 7, 0, 8)
+    d = ck.to_device(full)
+    out = torch.empty((15, P.n), dtype=torch.int32, device="cuda")
+    ctx24.ntt(out, d, 7, 0, 8, src_gap=17)
+    sync()
+    assert_same(ck.to_host(out), want, "ntt with gap")
+
+
+@pytest.mark.parametrize("mont_conv", [False, True])
+def test_intt_all_primes(ck, ctx24, oracle_rns24, mont_conv):
+    P = oracle_rns24
+    x = uniform(P, P.np_primes(24, 0, 8), SEED_BASE + 3)
+    want = P.intt_np(x, 24, 0, 8, mont_conv=mont_conv)
+    d = ck.to_device(x)
+    out = torch.empty_like(d)
+    ctx24.intt(out, d, 24, 0, 8, mont_conv=mont_conv)
+    sync()
+    assert_same(ck.to_host(out), want, "intt")
+
+
+@pytest.mark.parametrize("normalize", [False, True])
+def test_intt_mult_const(ck, ctx24, oracle_rns24, normalize):
+    P = oracle_rns24
+    pidx = P.np_primes(24, 0, 0)
+    x = uniform(P, pidx, SEED_BASE + 4)
+    rng = np.random.default_rng(4)
+    consts = [int(rng.integers(0, P.prime(i))) for i in pidx]
+    want = P.intt_mult_const(x, pidx, consts, normalize)
+    d = ck.to_device(x)
+    out = torch.empty_like(d)
+    ctx24.intt_mult_const(out, d, 24, 0, 0, consts, normalize)
+    sync()
+    assert_same(ck.to_host(out).view(np.int32), want, "intt_mult_const")
+
+
+def test_ntt_roundtrip_all_limbs(ck, ctx24, oracle_rns24):
+    P = oracle_rns24
+    x = uniform(P, P.np_primes(24, 0, 8), SEED_BASE + 5)
+    d = ck.to_device(x)
+    t = torch.empty_like(d)
+    ctx24.ntt(t, d, 24, 0, 8)
+    ctx24.intt(t, t, 24, 0, 8, mont_conv=False)
+    sync()
+    assert_same(ck.to_host(t), x, "INTT(NTT(x))")
+
+
+# ---------------------------------------------------------------------------
+# ModSwitchHandler
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("level", [23, 10, 7, 0])
+def test_modup(ck, ctx24, oracle_rns24, level):
+    P = oracle_rns24
+    nq = P.num_q(level)
+    x = uniform(P, P.level_primes(level), SEED_BASE + 6 + level)
+    want = P.modup(x, level)
+    beta, tot = want.shape[0], want.shape[1]
+    d = ck.to_device(x)
+    out = torch.zeros((beta * tot, P.n), dtype=torch.int32, device="cuda")
+    ctx24.modup(out, d, level)
+    sync()
+    assert_same(ck.to_host(out).reshape(beta, tot, P.n), want, f"modup level {level}")
+    assert nq + P.alpha == tot
+
+
+@pytest.mark.parametrize("level,kind", [(23, 0), (23, 1), (23, 2), (5, 0), (5, 2), (1, 1), (0, 0)])
+def test_moddown(ck, ctx24, oracle_rns24, level, kind):
+    P = oracle_rns24
+    nq = P.num_q(level)
+    nsrc = nq + (0 if kind == 1 else P.alpha)
+    srcs = [uniform(P, P.level_primes(level, with_aux=(kind != 1)), SEED_BASE + 40 + 2 * j)
+            for j in range(2)]
+    want = [P.moddown(s, level, kind) for s in srcs]
+    ds = [ck.to_device(s) for s in srcs]
+    outs = [torch.empty((w.shape[0], P.n), dtype=torch.int32, device="cuda") for w in want]
+    ctx24.moddown(outs, ds, level, kind)
+    sync()
+    for j in range(2):
+        assert srcs[j].shape[0] == nsrc
+        assert_same(ck.to_host(outs[j]), want[j], f"moddown kind {kind} level {level} poly {j}")
+
+
+# ---------------------------------------------------------------------------
+# ElementWiseHandler
+# ---------------------------------------------------------------------------
+def test_elementwise(ck, ctx24, oracle_rns24):
+    P = oracle_rns24
+    pidx = P.np_primes(24, 0, 8)
+    x = uniform(P, pidx, SEED_BASE + 60)
+    y = uniform(P, pidx, SEED_BASE + 61)
+    dx, dy = ck.to_device(x), ck.to_device(y)
+    out = torch.empty_like(dx)
+    np3 = (24, 0, 8)
+    for name, fn, want in (("add", ctx24.add, P.add(x, y, pidx)),
+                           ("sub", ctx24.sub, P.sub(x, y, pidx)),
+                           ("mult", ctx24.mult, P.mult(x, y, pidx))):
+        fn([out], [dx], [dy], np3)
+        sync()
+        assert_same(ck.to_host(out), want, name)
+    ctx24.neg([out], [dx], np3)
+    sync()
+    assert_same(ck.to_host(out), P.neg(x, pidx), "neg")
+    consts = np.array([np.uint32(P.p_prod_mont(i)) if i < 24 else np.uint32(7) for i in pidx],
+                      dtype=np.uint32)
+    ctx24.mult_const([out], [dx], ck.to_device(consts[None, :])[0], np3)
+    sync()
+    assert_same(ck.to_host(out), P.mult_const(x, consts, pidx), "mult_const")
+
+
+def test_tensor(ck, ctx24, oracle_rns24):
+    P = oracle_rns24
+    pidx = P.level_primes(23)
+    ins = [uniform(P, pidx, SEED_BASE + 70 + i) for i in range(4)]
+    want = P.tensor(*ins, pidx)
+    d = [ck.to_device(v) for v in ins]
+    outs = [torch.empty_like(d[0]) for _ in range(3)]
+    ctx24.tensor(*outs, *d, 24, 0)
+    sync()
+    for i in range(3):
+        assert_same(ck.to_host(outs[i]), want[i], f"tensor d{i}")
+
+
+@pytest.mark.parametrize("rot", [1, 1234, 32767, -1])
+def test_permute(ck, ctx24, oracle_rns24, rot):
+    P = oracle_rns24
+    pidx = P.level_primes(23)
+    x = uniform(P, pidx, SEED_BASE + 80)
+    want = P.permute(x, P.galois(rot))
+    d = ck.to_device(x)
+    out = torch.empty_like(d)
+    ctx24.permute([out], [d], rot, (24, 0, 0))
+    sync()
+    assert_same(ck.to_host(out), want, f"permute {rot}")
+
+
+# ---------------------------------------------------------------------------
+# Context evaluator
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def key24(oracle_rns24):
+    return full_key(oracle_rns24, SEED_BASE + 1000)
+
+
+@pytest.mark.parametrize("level,rescale", [(23, True), (23, False), (8, True), (1, True)])
+def test_hmult(ck, ctx24, oracle_rns24, key24, level, rescale):
+    P = oracle_rns24
+    pidx = P.level_primes(level)
+    ct1 = [uniform(P, pidx, SEED_BASE + 100 + i) for i in range(2)]
+    ct2 = [uniform(P, pidx, SEED_BASE + 110 + i) for i in range(2)]
+    want = P.hmult(level, ct1, ct2, key24, rescale=rescale)
+    nq_out = want[0].shape[0]
+    dk = dev_key(*key24)
+    d1 = [ck.to_device(v) for v in ct1]
+    d2 = [ck.to_device(v) for v in ct2]
+    out = [torch.empty((nq_out, P.n), dtype=torch.int32, device="cuda") for _ in range(2)]
+    ctx24.hmult(out, d1, d2, dk, level, rescale=rescale)
+    sync()
+    assert_same(ck.to_host(out[0]), want[0], f"hmult b level {level}")
+    assert_same(ck.to_host(out[1]), want[1], f"hmult a level {level}")
+
+
+def test_mult_key_no_moddown(ck, ctx24, oracle_rns24, key24):
+    P = oracle_rns24
+    level = 23
+    pidx = P.level_primes(level)
+    x = uniform(P, pidx, SEED_BASE + 120)
+    ab = uniform(P, pidx, SEED_BASE + 121)
+    aa = uniform(P, pidx, SEED_BASE + 122)
+    want = P.mult_key_no_moddown(level, x, key24[0], key24[1], ab, aa)
+    tot = P.num_q(level) + P.alpha
+    acc = [torch.empty((tot, P.n), dtype=torch.int32, device="cuda") for _ in range(2)]
+    ctx24.mult_key_no_moddown(acc[0], acc[1], ck.to_device(x), dev_key(*key24), level,
+                              ck.to_device(ab), ck.to_device(aa))
+    sync()
+    assert_same(ck.to_host(acc[0]), want[0], "accum b")
+    assert_same(ck.to_host(acc[1]), want[1], "accum a")
+
+
+@pytest.mark.parametrize("level,rot", [(23, 1), (23, 1234), (12, 32767), (3, 5)])
+def test_hrot(ck, ctx24, oracle_rns24, key24, level, rot):
+    P = oracle_rns24
+    pidx = P.level_primes(level)
+    ct = [uniform(P, pidx, SEED_BASE + 130 + i) for i in range(2)]
+    want = P.hrot(level, ct, key24, rot)
+    d = [ck.to_device(v) for v in ct]
+    out = [torch.empty_like(d[0]) for _ in range(2)]
+    ctx24.hrot(out, d, dev_key(*key24), rot, level)
+    sync()
+    assert_same(ck.to_host(out[0]), want[0], "hrot b")
+    assert_same(ck.to_host(out[1]), want[1], "hrot a")
+
+
+# ---------------------------------------------------------------------------
+# bootparam_40.json: terminal primes, uneven level steps, dnum 4
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("level", [28, 16, 2, 0])
+def test_boot40_modup(ck, ctx40, oracle_boot40, level):
+    P = oracle_boot40
+    x = uniform(P, P.level_primes(level), SEED_BASE + 200 + level)
+    want = P.modup(x, level)
+    beta, tot = want.shape[0], want.shape[1]
+    d = ck.to_device(x)
+    out = torch.zeros((beta * tot, P.n), dtype=torch.int32, device="cuda")
+    ctx40.modup(out, d, level)
+    sync()
+    assert_same(ck.to_host(out).reshape(beta, tot, P.n), want, f"boot40 modup {level}")
+
+
+# level transitions of bootparam_40: 28->27 drops terminal primes,
+# 3->2 (4,2)->(1,4) drops main primes and restores terminal ones,
+# 2->1 (1,4)->(3,0) drops terminal primes and restores main ones
+@pytest.mark.parametrize("level,kind", [(28, 0), (28, 1), (28, 2), (27, 2), (17, 2),
+                                        (3, 1), (3, 2), (2, 1), (2, 2), (1, 2), (5, 0)])
+def test_boot40_moddown(ck, ctx40, oracle_boot40, level, kind):
+    P = oracle_boot40
+    srcs = [uniform(P, P.level_primes(level, with_aux=(kind != 1)), SEED_BASE + 300 + 2 * j)
+            for j in range(2)]
+    want = [P.moddown(s, level, kind) for s in srcs]
+    ds = [ck.to_device(s) for s in srcs]
+    outs = [torch.empty((w.shape[0], P.n), dtype=torch.int32, device="cuda") for w in want]
+    ctx40.moddown(outs, ds, level, kind)
+    sync()
+    for j in range(2):
+        assert_same(ck.to_host(outs[j]), want[j], f"boot40 moddown kind {kind} level {level}")
+
+
+@pytest.mark.parametrize("level", [28, 9])
+def test_boot40_hmult(ck, ctx40, oracle_boot40, level):
+    P = oracle_boot40
+    key = full_key(P, SEED_BASE + 2000)
+    pidx = P.level_primes(level)
+    ct1 = [uniform(P, pidx, SEED_BASE + 400 + i) for i in range(2)]
+    ct2 = [uniform(P, pidx, SEED_BASE + 410 + i) for i in range(2)]
+    want = P.hmult(level, ct1, ct2, key, rescale=True)
+    d1 = [ck.to_device(v) for v in ct1]
+    d2 = [ck.to_device(v) for v in ct2]
+    out = [torch.empty((want[0].shape[0], P.n), dtype=torch.int32, device="cuda") for _ in range(2)]
+    ctx40.hmult(out, d1, d2, dev_key(*key), level, rescale=True)
+    sync()
+    assert_same(ck.to_host(out[0]), want[0], "boot40 hmult b")
+    assert_same(ck.to_host(out[1]), want[1], "boot40 hmult a")
