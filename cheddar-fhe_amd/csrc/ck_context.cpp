@@ -124,13 +124,19 @@ int upload(Context &c, T **dptr, const std::vector<T> &h) {
 // c1[i] = ((prod_{j!=i} s_j) * N)^-1 * prod(restore) mod s_i, returned in
 // plain form (times R^-1 because the reference applies it with a Montgomery
 // product); table[m][i] = centred((prod_{j!=i} s_j) * R^2 mod t_m).
+//
+// ftab holds the same products for the exact FP64 BaseConv fused into the
+// forward NTT: T'[m][i] = (prod_{j!=i} s_j) * R mod t_m (one Montgomery
+// factor: the fused form adds no R^-1) split as hi * 2^16 + lo.
 void bconv_constants(const Context &c, const std::vector<int> &src,
                      const std::vector<int> &dst,
                      const std::vector<int> &restore,
-                     std::vector<uint32_t> &c1, std::vector<int32_t> &table) {
+                     std::vector<uint32_t> &c1, std::vector<int32_t> &table,
+                     std::vector<double> &ftab) {
   const size_t s = src.size(), d = dst.size();
   c1.assign(s, 0);
   table.assign(s * d, 0);
+  ftab.assign(2 * s * d, 0.0);
   for (size_t i = 0; i < s; i++) {
     uint32_t mod = c.primes[src[i]];
     uint32_t v = 1;
@@ -147,8 +153,10 @@ void bconv_constants(const Context &c, const std::vector<int> &src,
       uint32_t v = 1;
       for (size_t j = 0; j < s; j++)
         if (j != i) v = h_mulmod(v, c.primes[src[j]] % mod, mod);
-      v = h_to_mont(h_to_mont(v, mod), mod);
-      table[m * s + i] = centred(v, mod);
+      const uint32_t v1 = h_to_mont(v, mod);
+      ftab[2 * (m * s + i)] = (double)(v1 >> 16);
+      ftab[2 * (m * s + i) + 1] = (double)(v1 & 0xFFFFu);
+      table[m * s + i] = centred(h_to_mont(v1, mod), mod);
     }
   }
 }
@@ -195,7 +203,8 @@ int build_moddown(Context &c, int level, int kind, ModDownPlan &mp) {
     restore.push_back(dst_prime[j]);
 
   std::vector<int32_t> table;
-  bconv_constants(c, src_prime, dst_prime, restore, mp.c1, table);
+  std::vector<double> ftab;
+  bconv_constants(c, src_prime, dst_prime, restore, mp.c1, table, ftab);
   mp.src_len = (int)src_limb.size();
   mp.dst_len = next_num_q;
   mp.src_limb = src_limb;
@@ -225,7 +234,7 @@ int build_moddown(Context &c, int level, int kind, ModDownPlan &mp) {
     mp.inv_prod[y] = h_invmod(ip, mod);
   }
   mp.use_pad = !restore.empty();
-  int rc = upload(c, &mp.d_table, table);
+  int rc = upload(c, &mp.d_ftab, ftab);
   if (rc) return rc;
   mp.valid = true;
   return kOk;
@@ -269,9 +278,10 @@ int build_level(Context &c, int level) {
     }
     std::vector<uint32_t> c1;
     std::vector<int32_t> table;
-    bconv_constants(c, src, dst, {}, c1, table);
+    std::vector<double> ftab;
+    bconv_constants(c, src, dst, {}, c1, table, ftab);
     for (int t = s0; t < s1; t++) lp.modup_c1[t] = c1[t - s0];
-    int rc = upload(c, &gr.d_table, table);
+    int rc = upload(c, &gr.d_ftab, ftab);
     if (rc) return rc;
   }
   int rc = build_moddown(c, level, 0, lp.moddown);

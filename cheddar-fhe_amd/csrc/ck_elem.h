@@ -30,8 +30,19 @@ struct KeyAccumArgs {
   const uint32_t *key_a[kMaxDnum];
   int group[kMaxDnum];
   int ngroups;
+  // ModUp source (NTT domain, num_q limbs): a group's own limbs [own0, own1)
+  // are read from here instead of the ModUp output (ModUp copies them
+  // unchanged, ModSwitch.cu:466-470)
+  const uint32_t *x;
+  int own0[kMaxDnum];
+  int own1[kMaxDnum];
   const uint32_t *add_b;  // num_q limbs or null
   const uint32_t *add_a;
+  // HMult form: tensor = (b1, a1, b2, a2); the key-switched polynomial is
+  // d2 = a1 (*) a2 (own limbs formed here) and the fold adds P * d0,
+  // P * d1 with d0 = b1 (*) b2, d1 = b1 (*) a2 + a1 (*) b2
+  const uint32_t *tensor[4];
+  int use_tensor;
   const uint32_t *p_prod;  // by global prime
   uint32_t *out_b;
   uint32_t *out_a;

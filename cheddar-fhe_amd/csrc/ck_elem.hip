@@ -142,9 +142,33 @@ __global__ __launch_bounds__(kThreads) void key_accum_kernel(
   uint32_t rb[4] = {0, 0, 0, 0}, ra[4] = {0, 0, 0, 0};
   const size_t my = (size_t)y * 65536 + k;
   const size_t kp = (size_t)p * 65536 + k;
+  // HMult form: Tensor (ElementWise.cu:504-546) of the two input ciphertexts
+  // on this q limb
+  uint32_t d0[4] = {0, 0, 0, 0}, d1[4] = {0, 0, 0, 0}, d2[4] = {0, 0, 0, 0};
+  const bool tens = A.use_tensor && y < np.num_q;
+  if (tens) {
+    const uint4 b1 = ld4(A.tensor[0] + my), a1 = ld4(A.tensor[1] + my);
+    const uint4 b2 = ld4(A.tensor[2] + my), a2 = ld4(A.tensor[3] + my);
+    const uint32_t B1[4] = {b1.x, b1.y, b1.z, b1.w};
+    const uint32_t A1[4] = {a1.x, a1.y, a1.z, a1.w};
+    const uint32_t B2[4] = {b2.x, b2.y, b2.z, b2.w};
+    const uint32_t A2[4] = {a2.x, a2.y, a2.z, a2.w};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      d0[i] = mont_mul(B1[i], B2[i], q, qinv);
+      d1[i] = add_mod(mont_mul(B1[i], A2[i], q, qinv),
+                      mont_mul(A1[i], B2[i], q, qinv), q);
+      d2[i] = mont_mul(A1[i], A2[i], q, qinv);
+    }
+  }
   for (int gi = 0; gi < A.ngroups; gi++) {
     const int g = A.group[gi];
-    uint4 mu = ld4(A.modup + (size_t)g * tot_stride + my);
+    const bool own = (y >= A.own0[g] && y < A.own1[g]);
+    uint4 mu;
+    if (own)
+      mu = tens ? make_uint4(d2[0], d2[1], d2[2], d2[3]) : ld4(A.x + my);
+    else
+      mu = ld4(A.modup + (size_t)g * tot_stride + my);
     uint4 kb = ld4(A.key_b[g] + kp);
     uint4 ka = ld4(A.key_a[g] + kp);
     const uint32_t M[4] = {mu.x, mu.y, mu.z, mu.w};
@@ -156,7 +180,14 @@ __global__ __launch_bounds__(kThreads) void key_accum_kernel(
       ra[i] = add_mod(ra[i], mont_mul(M[i], KA[i], q, qinv), q);
     }
   }
-  if (y < np.num_q) {
+  if (tens) {
+    const uint32_t pp = A.p_prod[p];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      rb[i] = add_mod(rb[i], mont_mul(pp, d0[i], q, qinv), q);
+      ra[i] = add_mod(ra[i], mont_mul(pp, d1[i], q, qinv), q);
+    }
+  } else if (y < np.num_q) {
     const uint32_t pp = A.p_prod[p];
     if (A.add_b) {
       uint4 v = ld4(A.add_b + my);

@@ -41,14 +41,24 @@ enum NttFlags : int {
   kNttModDownEpilogue = 2,  // forward: out = (src2 * c2 - NTT(x)) * c3
   kNttEpiloguePad = 4,      // forward: apply c2 in the epilogue
   kInttCentred = 8,         // inverse: centred int32 output
+  kInttLoadProduct = 16,    // inverse: input = src (*) src2 (Montgomery)
+  kNttBconvInput = 32,      // forward: input = BaseConv of bsrc (see below)
 };
 
 struct NttLaunch {
   // up to four base pointers per role; limb i uses src[sel & 3],
-  // dst[(sel >> 2) & 3], src2[(sel >> 4) & 3]
+  // dst[(sel >> 2) & 3], src2[(sel >> 4) & 3], bsrc[(sel >> 6) & 3]
   const uint32_t *src[4];
   uint32_t *dst[4];
   const uint32_t *src2[4];
+  // kNttBconvInput: the transform input of limb i is
+  //   sum_j bsrc[j] * T_j  mod q, T_j = B[btab_off[i] + 2j] * 2^16 +
+  //   B[btab_off[i] + 2j + 1], B = btab[(sel >> 6) & 3]
+  // over bs[(sel >> 6) & 3] contiguous centred int32 source limbs
+  // (BaseConv, exact)
+  const int32_t *bsrc[4];
+  const double *btab[4];
+  int bs[4];
   int n;
   int flags;
   uint8_t prime[kMaxLaunchLimbs];
@@ -56,23 +66,14 @@ struct NttLaunch {
   uint16_t src_limb[kMaxLaunchLimbs];
   uint16_t dst_limb[kMaxLaunchLimbs];
   int16_t src2_limb[kMaxLaunchLimbs];
+  uint16_t btab_off[kMaxLaunchLimbs];
   uint32_t c1[kMaxLaunchLimbs];  // plain-form constants (already carry R^-1)
   uint32_t c2[kMaxLaunchLimbs];
   uint32_t c3[kMaxLaunchLimbs];
 };
 
-// Fast base conversion launch: dst limb m (global prime dst_prime[m], at
-// dst_limb[m] of dst) = ReduceMontgomery(sum_i src[i] * table[m * s + i]).
-constexpr int kMaxBconvDst = 112;
-constexpr int kMaxBconvSrc = 16;
-struct BconvLaunch {
-  const int32_t *src[4];  // per poly (grid.z): s contiguous limbs, centred
-  uint32_t *dst[4];       // per poly
-  const int32_t *table;   // d x s, device
-  int s, d;
-  uint8_t dst_prime[kMaxBconvDst];
-  uint16_t dst_limb[kMaxBconvDst];
-};
+constexpr int kMaxBconvSrc = 16;  // source limbs of one BaseConv
+constexpr int kMaxBconvDst = kMaxLaunchLimbs;
 
 // Tables of one ModDown flavour (ModDown, Rescale, ModDownAndRescale) at one
 // level, mirroring ModSwitchHandler's constants (ModSwitch.cu:233-411).
@@ -87,14 +88,14 @@ struct ModDownPlan {
   std::vector<uint32_t> pad;   // epilogue padding (plain)
   std::vector<uint32_t> inv_prod;  // epilogue (prod src)^-1 (plain)
   bool use_pad = false;
-  int32_t *d_table = nullptr;  // dst_len x src_len
+  double *d_ftab = nullptr;    // dst_len x src_len x {hi, lo}, see NttLaunch
 };
 
 struct ModUpGroup {
   int s0 = 0, s1 = 0;           // own limb range [s0, s1) within the level
   std::vector<int> dst_limb;    // limb index in the (num_q + alpha) output
   std::vector<int> dst_prime;   // global prime
-  int32_t *d_table = nullptr;   // dst x src
+  double *d_ftab = nullptr;     // dst x src x {hi, lo}
 };
 
 struct LevelPlan {
@@ -154,7 +155,5 @@ uint32_t h_to_mont(uint32_t a, uint32_t q);
 // ---- launchers ----
 void launch_ntt_fwd(const Context &ctx, const NttLaunch &L, hipStream_t st);
 void launch_intt(const Context &ctx, const NttLaunch &L, hipStream_t st);
-void launch_bconv(const Context &ctx, const BconvLaunch &B, int npoly,
-                  hipStream_t st);
 
 }  // namespace ck

@@ -8,52 +8,48 @@
 // Gentleman-Sande mirror with psi^-1 and no N^-1 factor; the per-limb
 // constant fused after it carries N^-1 and whatever else the caller folds.
 //
-// MI355X mapping. A limb of 2^16 u32 is viewed as 256 rows x 256 columns.
-//   * "column" kernels run the 8 stages whose butterflies pair rows
-//     (s = 0..7 forward, u = 8..15 inverse). A 256-thread workgroup owns a
-//     tile of 256 rows x 16 columns; each thread holds 16 values of one
-//     column and runs two radix-16 register phases separated by one LDS
-//     transpose.
-//   * "row" kernels run the 8 stages inside a 256-element row. A
-//     workgroup owns 16 rows; 16 threads per row, 16 values per thread.
-// Every butterfly uses the exact FP64 modular product of ck_arith.h (full
-// rate FMA on CDNA4) with lazy ranges: values stay integer doubles well
-// below 2^48 inside a kernel and are reduced to canonical u32 on store, so
-// HBM traffic stays at 4 B per coefficient per pass.
+// MI355X mapping. A limb of 2^16 u32 is a 256 x 256 matrix (row = 256
+// consecutive coefficients).
+//   * row kernels (forward stages 8..15, inverse stages 0..7): a 256-thread
+//     workgroup owns 16 consecutive rows = one contiguous 16 KiB block,
+//     moved with 16-byte loads/stores that cover whole cache lines.
+//   * column kernels (forward stages 0..7, inverse 8..15): a 512-thread
+//     workgroup owns all 256 rows x 32 columns (128 B per row segment).
+// Each thread runs two radix-16 register phases on 16 values. Global data
+// move through an int32 LDS staging tile with 16-byte accesses; the hand-off
+// between the two phases keeps the lazy doubles in an LDS tile aliased with
+// the staging tile (row tiles use an XOR swizzle so both phase layouts are
+// free of bank conflicts). Twiddles are read once per workgroup phase as u32
+// and issued before the data loads. Butterflies use the exact FP64 modular
+// product of ck_arith.h with lazy ranges (integer doubles below 2^40); the
+// intermediate between the two passes of a transform is stored as a lazy
+// int32 in (-q/2 - 2, q/2 + 2) and only final outputs are made canonical.
+// Each thread writes a phase's results back to exactly the LDS words it read
+// for that phase, so only the hand-offs between threads need a barrier.
 #include "ck_internal.h"
 
 namespace ck {
 namespace {
 
-constexpr int kLogN = 16;
-constexpr int kN = 1 << kLogN;
-constexpr int kThreads = 256;
-constexpr int kTile = 16;  // columns per column-tile, rows per row-tile
-constexpr int kTilesPerLimb = 256 / kTile;
-constexpr int kLdsStride = 256 + 16;  // padded row of the row kernels
-constexpr int kColStride = kTile + 1; // padded row of the column kernels
+constexpr int kN = 65536;
+constexpr int kRowThreads = 256;
+constexpr int kRowTile = 16;                  // rows per row-kernel workgroup
+constexpr int kRowTiles = 256 / kRowTile;     // per limb
+constexpr int kColThreads = 512;
+constexpr int kColTile = 32;                  // columns per column workgroup
+constexpr int kColTiles = 256 / kColTile;     // per limb
 
-struct Tw16 {
-  double w[15];
-  double wq[15];
-};
-
-__device__ __forceinline__ void set_tw(Tw16 &t, int i, uint32_t w, double qr) {
-  double d = (double)w;
-  t.w[i] = d;
-  t.wq[i] = d * qr;
-}
-
-// forward radix-16 (4 CT stages), element index bits high to low;
-// twiddle slot (1 << s) - 1 + blk for sub-stage s
-__device__ __forceinline__ void ct16(double x[16], const Tw16 &t, double q) {
+// forward radix-16 (4 CT stages), element index bits high to low; twiddle
+// slot (1 << s) - 1 + blk for sub-stage s
+__device__ __forceinline__ void ct16(double x[16], const uint32_t tw[15],
+                                     double q, double qr) {
 #pragma unroll
   for (int s = 0; s < 4; s++) {
     const int half = 8 >> s;
 #pragma unroll
     for (int blk = 0; blk < (1 << s); blk++) {
-      const double w = t.w[(1 << s) - 1 + blk];
-      const double wq = t.wq[(1 << s) - 1 + blk];
+      const double w = (double)tw[(1 << s) - 1 + blk];
+      const double wq = w * qr;
 #pragma unroll
       for (int j = 0; j < half; j++) {
         double &a = x[blk * 2 * half + j];
@@ -66,16 +62,17 @@ __device__ __forceinline__ void ct16(double x[16], const Tw16 &t, double q) {
   }
 }
 
-// inverse radix-16 (4 GS stages), element index bits low to high;
-// twiddle slot 16 - (16 >> u) + blk for sub-stage u
-__device__ __forceinline__ void gs16(double x[16], const Tw16 &t, double q) {
+// inverse radix-16 (4 GS stages), element index bits low to high; twiddle
+// slot 16 - (16 >> u) + blk for sub-stage u
+__device__ __forceinline__ void gs16(double x[16], const uint32_t tw[15],
+                                     double q, double qr) {
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const int half = 1 << u;
 #pragma unroll
     for (int blk = 0; blk < (8 >> u); blk++) {
-      const double w = t.w[16 - (16 >> u) + blk];
-      const double wq = t.wq[16 - (16 >> u) + blk];
+      const double w = (double)tw[16 - (16 >> u) + blk];
+      const double wq = w * qr;
 #pragma unroll
       for (int j = 0; j < half; j++) {
         double &a = x[blk * 2 * half + j];
@@ -88,102 +85,237 @@ __device__ __forceinline__ void gs16(double x[16], const Tw16 &t, double q) {
   }
 }
 
+// Row-tile LDS index: row rl, in-row offset b, XOR swizzle of the low nibble
+// by the high nibble (both b = t + 16k and b = 16t + j patterns hit 16
+// distinct banks).
+__device__ __forceinline__ int rlds(int rl, int b) {
+  return rl * 256 + (b ^ ((b >> 4) & 15));
+}
+
+struct LimbView {
+  const uint32_t *src;
+  uint32_t *dst;
+  int p;
+};
+
+__device__ __forceinline__ LimbView limb_view(const NttLaunch &L, int y) {
+  const int sel = L.sel[y];
+  return LimbView{L.src[sel & 3] + (size_t)L.src_limb[y] * kN,
+                  L.dst[(sel >> 2) & 3] + (size_t)L.dst_limb[y] * kN,
+                  L.prime[y]};
+}
+
+// 16 KiB contiguous row tile <-> LDS, 16-byte accesses
+__device__ __forceinline__ void tile_rows_to_lds(const uint32_t *g, int32_t *lds) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int e = 1024 * i + 4 * threadIdx.x;
+    const uint4 v = *reinterpret_cast<const uint4 *>(g + e);
+    const int rl = e >> 8, b = e & 255;
+    lds[rlds(rl, b)] = (int32_t)v.x;
+    lds[rlds(rl, b + 1)] = (int32_t)v.y;
+    lds[rlds(rl, b + 2)] = (int32_t)v.z;
+    lds[rlds(rl, b + 3)] = (int32_t)v.w;
+  }
+}
+
+__device__ __forceinline__ void lds_to_tile_rows(const int32_t *lds, uint32_t *g) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int e = 1024 * i + 4 * threadIdx.x;
+    const int rl = e >> 8, b = e & 255;
+    uint4 v;
+    v.x = (uint32_t)lds[rlds(rl, b)];
+    v.y = (uint32_t)lds[rlds(rl, b + 1)];
+    v.z = (uint32_t)lds[rlds(rl, b + 2)];
+    v.w = (uint32_t)lds[rlds(rl, b + 3)];
+    *reinterpret_cast<uint4 *>(g + e) = v;
+  }
+}
+
+// column tile (256 rows x 32 columns starting at column c0) <-> LDS [256][32]
+__device__ __forceinline__ void tile_cols_to_lds(const uint32_t *g, int c0,
+                                                 int32_t *lds) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int idx = 512 * i + threadIdx.x;  // 16-byte chunk index
+    const int r = idx >> 3, c4 = (idx & 7) * 4;
+    const uint4 v = *reinterpret_cast<const uint4 *>(g + r * 256 + c0 + c4);
+    *reinterpret_cast<int4 *>(lds + r * 32 + c4) =
+        make_int4((int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w);
+  }
+}
+
+// BaseConv computed straight into the column tile (ModSwitchMatrixMult,
+// src/core/ModSwitch.cu:23-171, in exact FP64): for each coefficient,
+// y = sum_j v_j * T_j mod t over bs centred int32 source limbs, with
+// T_j = hi_j * 2^16 + lo_j so every partial sum stays an exact double
+// (|v| < 2^30: |sum v hi| < 2^49, |sum v lo| < 2^50 for bs <= 16).
+__device__ __forceinline__ void bconv_cols_to_lds(const int32_t *bsrc, int bs,
+                                                  const double *tab, int c0,
+                                                  const ModF &m,
+                                                  int32_t *lds) {
+  double th[kMaxBconvSrc], tl[kMaxBconvSrc];
+#pragma unroll
+  for (int j = 0; j < kMaxBconvSrc; j++) {
+    if (j < bs) {
+      th[j] = tab[2 * j];
+      tl[j] = tab[2 * j + 1];
+    }
+  }
+  for (int i = 0; i < 4; i++) {
+    const int idx = 512 * i + threadIdx.x;
+    const int r = idx >> 3, c4 = (idx & 7) * 4;
+    const int32_t *p = bsrc + r * 256 + c0 + c4;
+    double sh[4] = {0.0, 0.0, 0.0, 0.0}, sl[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < kMaxBconvSrc; j++) {
+      if (j < bs) {
+        const int4 v = *reinterpret_cast<const int4 *>(p + (size_t)j * kN);
+        const double vv[4] = {(double)v.x, (double)v.y, (double)v.z,
+                              (double)v.w};
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          sh[e] = __builtin_fma(vv[e], th[j], sh[e]);
+          sl[e] = __builtin_fma(vv[e], tl[j], sl[e]);
+        }
+      }
+    }
+    int4 o;
+    int32_t *op = reinterpret_cast<int32_t *>(&o);
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const double big = sh[e] * 65536.0;  // exact (power-of-two scale)
+      const double k = __builtin_rint(big * m.qinv);
+      op[e] = (int32_t)reduce_f(__builtin_fma(-k, m.q, big) + sl[e], m);
+    }
+    *reinterpret_cast<int4 *>(lds + r * 32 + c4) = o;
+  }
+}
+
+__device__ __forceinline__ void lds_to_tile_cols(const int32_t *lds, uint32_t *g,
+                                                 int c0) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int idx = 512 * i + threadIdx.x;
+    const int r = idx >> 3, c4 = (idx & 7) * 4;
+    const int4 v = *reinterpret_cast<const int4 *>(lds + r * 32 + c4);
+    *reinterpret_cast<uint4 *>(g + r * 256 + c0 + c4) =
+        make_uint4((uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z, (uint32_t)v.w);
+  }
+}
+
+// Final value of an element: canonical u32, or lazy int32 in
+// (-q/2 - 2, q/2 + 2) for a pass-1 intermediate that pass 2 reads back.
+__device__ __forceinline__ int32_t lazy_i32(double x, const ModF &m) {
+  return (int32_t)reduce_f(x, m);
+}
+
 // ---------------------------------------------------------------------------
-// Forward, column stages s = 0..7
+// Forward, column stages s = 0..7 -> lazy int32 intermediate
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void ntt_fwd_cols(
+__global__ __launch_bounds__(kColThreads) void ntt_fwd_cols(
     const NttLaunch L, const PrimeTab P, const uint32_t *__restrict__ tw) {
-  __shared__ double lds[256 * kColStride];
-  const int y = blockIdx.y;
-  const int p = L.prime[y];
-  const double q = P.qd[p], qr = P.qr[p];
+  // int32 staging tile [256][32] aliased with the double exchange tile
+  __shared__ double ldsd[256 * kColTile];
+  int32_t *lds = reinterpret_cast<int32_t *>(ldsd);
+  // XCD-aware: workgroups are dealt round-robin to the 8 XCDs, so column
+  // tile (blockIdx.x & 7) of every limb runs on the same XCD and the source
+  // tiles a BaseConv input shares stay in that XCD's L2.
+  const int y = blockIdx.x >> 3;
+  const int c0 = (blockIdx.x & 7) * kColTile;
+  const LimbView v = limb_view(L, y);
+  const double q = P.qd[v.p], qr = P.qr[v.p];
   const ModF m{q, qr};
-  const uint32_t *__restrict__ src = L.src[L.sel[y] & 3] + (size_t)L.src_limb[y] * kN;
-  uint32_t *__restrict__ dst = L.dst[(L.sel[y] >> 2) & 3] + (size_t)L.dst_limb[y] * kN;
-  const uint32_t *__restrict__ w = tw + (size_t)p * kN;
-  const int cc = threadIdx.x & (kTile - 1);
-  const int rb = threadIdx.x >> 4;
-  const int c = blockIdx.x * kTile + cc;
+  const uint32_t *__restrict__ w = tw + (size_t)v.p * kN;
+  const int cc = threadIdx.x & 31;
+  const int rb = threadIdx.x >> 5;  // 0..15
+
+  // phase 1 twiddles psi_rev[2^s + blk] (uniform), phase 2 psi_rev[2^(4+s)
+  // + rb 2^s + blk]
+  uint32_t t1[15], t2[15];
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int blk = 0; blk < (1 << s); blk++) {
+      t1[(1 << s) - 1 + blk] = w[(1 << s) + blk];
+      t2[(1 << s) - 1 + blk] = w[(16 << s) + (rb << s) + blk];
+    }
+  if (L.flags & kNttBconvInput)
+    bconv_cols_to_lds(L.bsrc[(L.sel[y] >> 6) & 3], L.bs[(L.sel[y] >> 6) & 3],
+                      L.btab[(L.sel[y] >> 6) & 3] + L.btab_off[y], c0, m, lds);
+  else
+    tile_cols_to_lds(v.src, c0, lds);
+  __syncthreads();
 
   double x[16];
 #pragma unroll
-  for (int k = 0; k < 16; k++) x[k] = (double)src[(rb + 16 * k) * 256 + c];
+  for (int k = 0; k < 16; k++) x[k] = (double)lds[(rb + 16 * k) * 32 + cc];
   if (L.flags & kNttPreMul) {
     const double cv = (double)L.c1[y], cq = cv * qr;
 #pragma unroll
     for (int k = 0; k < 16; k++) x[k] = mulmod_f(x[k], cv, cq, q);
   }
-  Tw16 t;
-  // phase 1: rows rb + 16k, stages 0..3, twiddle psi_rev[2^s + blk]
+  ct16(x, t1, q, qr);
+  __syncthreads();  // staging words are reused by the double tile
 #pragma unroll
-  for (int s = 0; s < 4; s++)
-#pragma unroll
-    for (int blk = 0; blk < (1 << s); blk++)
-      set_tw(t, (1 << s) - 1 + blk, w[(1 << s) + blk], qr);
-  ct16(x, t, q);
-#pragma unroll
-  for (int k = 0; k < 16; k++) lds[(rb + 16 * k) * kColStride + cc] = x[k];
+  for (int k = 0; k < 16; k++) ldsd[(rb + 16 * k) * 32 + cc] = x[k];
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < 16; j++) x[j] = lds[(rb * 16 + j) * kColStride + cc];
-  // phase 2: rows rb*16 + j, stages 4..7,
-  // twiddle psi_rev[2^(4+s) + rb*2^s + blk]
+  for (int j = 0; j < 16; j++) x[j] = ldsd[(rb * 16 + j) * 32 + cc];
+  ct16(x, t2, q, qr);
+  __syncthreads();
 #pragma unroll
-  for (int s = 0; s < 4; s++)
-#pragma unroll
-    for (int blk = 0; blk < (1 << s); blk++)
-      set_tw(t, (1 << s) - 1 + blk, w[(16 << s) + (rb << s) + blk], qr);
-  ct16(x, t, q);
-#pragma unroll
-  for (int j = 0; j < 16; j++) dst[(rb * 16 + j) * 256 + c] = canon_f(x[j], m);
+  for (int j = 0; j < 16; j++) lds[(rb * 16 + j) * 32 + cc] = lazy_i32(x[j], m);
+  __syncthreads();
+  lds_to_tile_cols(lds, v.dst, c0);
 }
 
 // ---------------------------------------------------------------------------
 // Forward, row stages s = 8..15 (+ optional ModDown epilogue)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void ntt_fwd_rows(
+__global__ __launch_bounds__(kRowThreads) void ntt_fwd_rows(
     const NttLaunch L, const PrimeTab P, const uint32_t *__restrict__ tw) {
-  __shared__ double lds[kTile * kLdsStride];
+  __shared__ double ldsd[kRowTile * 256];
+  int32_t *lds = reinterpret_cast<int32_t *>(ldsd);
   const int y = blockIdx.y;
-  const int p = L.prime[y];
-  const double q = P.qd[p], qr = P.qr[p];
+  const LimbView v = limb_view(L, y);
+  const double q = P.qd[v.p], qr = P.qr[v.p];
   const ModF m{q, qr};
-  const int rl = threadIdx.x >> 4;  // row within tile
+  const uint32_t *__restrict__ w = tw + (size_t)v.p * kN;
+  const int rl = threadIdx.x >> 4;
   const int t16 = threadIdx.x & 15;
-  const int r = blockIdx.x * kTile + rl;  // row within limb
-  const uint32_t *__restrict__ src =
-      L.src[L.sel[y] & 3] + (size_t)L.src_limb[y] * kN + r * 256;
-  uint32_t *__restrict__ dst = L.dst[(L.sel[y] >> 2) & 3] + (size_t)L.dst_limb[y] * kN + r * 256;
-  const uint32_t *__restrict__ w = tw + (size_t)p * kN;
-  double *row = lds + rl * kLdsStride;
+  const int r = blockIdx.x * kRowTile + rl;  // row within the limb
+  const size_t tile = (size_t)blockIdx.x * kRowTile * 256;
+
+  // phase 1 twiddles psi_rev[2^(8+s) + r 2^s + blk], phase 2
+  // psi_rev[2^(12+s) + r 2^(4+s) + t16 2^s + blk]
+  uint32_t t1[15], t2[15];
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int blk = 0; blk < (1 << s); blk++) {
+      t1[(1 << s) - 1 + blk] = w[(256 << s) + (r << s) + blk];
+      t2[(1 << s) - 1 + blk] =
+          w[(4096 << s) + (r << (4 + s)) + (t16 << s) + blk];
+    }
+  // the pass-1 intermediate is lazy signed int32
+  tile_rows_to_lds(v.src + tile, lds);
+  __syncthreads();
 
   double x[16];
 #pragma unroll
-  for (int k = 0; k < 16; k++) x[k] = (double)src[t16 + 16 * k];
-  Tw16 t;
-  // phase 1: b = t16 + 16k, stages 8..11, twiddle psi_rev[2^(8+s) + r*2^s +
-  // blk]
-#pragma unroll
-  for (int s = 0; s < 4; s++)
-#pragma unroll
-    for (int blk = 0; blk < (1 << s); blk++)
-      set_tw(t, (1 << s) - 1 + blk, w[(256 << s) + (r << s) + blk], qr);
-  ct16(x, t, q);
-#pragma unroll
-  for (int k = 0; k < 16; k++) row[t16 + 17 * k] = x[k];
+  for (int k = 0; k < 16; k++) x[k] = (double)lds[rlds(rl, t16 + 16 * k)];
+  ct16(x, t1, q, qr);
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < 16; j++) x[j] = row[17 * t16 + j];
-  // phase 2: b = t16*16 + j, stages 12..15,
-  // twiddle psi_rev[2^(12+s) + r*2^(4+s) + t16*2^s + blk]
+  for (int k = 0; k < 16; k++) ldsd[rlds(rl, t16 + 16 * k)] = x[k];
+  __syncthreads();
 #pragma unroll
-  for (int s = 0; s < 4; s++)
-#pragma unroll
-    for (int blk = 0; blk < (1 << s); blk++)
-      set_tw(t, (1 << s) - 1 + blk,
-             w[(4096 << s) + (r << (4 + s)) + (t16 << s) + blk], qr);
-  ct16(x, t, q);
+  for (int j = 0; j < 16; j++) x[j] = ldsd[rlds(rl, 16 * t16 + j)];
+  ct16(x, t2, q, qr);
 
+  // this thread now holds the 16 consecutive outputs b = 16 t16 + j of row r
   uint32_t out[16];
   if (L.flags & kNttModDownEpilogue) {
     // NTTPhase2ForModDown (NTT.cu:456-601): out = (src2 * pad - y) * ip
@@ -191,16 +323,15 @@ __global__ __launch_bounds__(kThreads) void ntt_fwd_rows(
     const double ip = (double)L.c3[y], ipq = ip * qr;
     double a2[16];
     if (s2 >= 0) {
-      const uint32_t *__restrict__ sp =
-          L.src2[(L.sel[y] >> 4) & 3] + (size_t)s2 * kN + r * 256 + t16 * 16;
-      const uint4 *v4 = reinterpret_cast<const uint4 *>(sp);
+      const uint32_t *__restrict__ sp = L.src2[(L.sel[y] >> 4) & 3] +
+                                        (size_t)s2 * kN + r * 256 + t16 * 16;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        uint4 v = v4[i];
-        a2[4 * i + 0] = (double)v.x;
-        a2[4 * i + 1] = (double)v.y;
-        a2[4 * i + 2] = (double)v.z;
-        a2[4 * i + 3] = (double)v.w;
+        const uint4 u = reinterpret_cast<const uint4 *>(sp)[i];
+        a2[4 * i + 0] = (double)u.x;
+        a2[4 * i + 1] = (double)u.y;
+        a2[4 * i + 2] = (double)u.z;
+        a2[4 * i + 3] = (double)u.w;
       }
       if (L.flags & kNttEpiloguePad) {
         const double pv = (double)L.c2[y], pq = pv * qr;
@@ -218,7 +349,7 @@ __global__ __launch_bounds__(kThreads) void ntt_fwd_rows(
 #pragma unroll
     for (int j = 0; j < 16; j++) out[j] = canon_f(x[j], m);
   }
-  uint4 *o4 = reinterpret_cast<uint4 *>(dst + t16 * 16);
+  uint4 *o4 = reinterpret_cast<uint4 *>(v.dst + (size_t)r * 256 + t16 * 16);
 #pragma unroll
   for (int i = 0; i < 4; i++)
     o4[i] = make_uint4(out[4 * i], out[4 * i + 1], out[4 * i + 2],
@@ -226,118 +357,134 @@ __global__ __launch_bounds__(kThreads) void ntt_fwd_rows(
 }
 
 // ---------------------------------------------------------------------------
-// Inverse, row stages u = 0..7
+// Inverse, row stages u = 0..7 -> lazy int32 intermediate
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void intt_rows(
+__global__ __launch_bounds__(kRowThreads) void intt_rows(
     const NttLaunch L, const PrimeTab P, const uint32_t *__restrict__ tw) {
-  __shared__ double lds[kTile * kLdsStride];
+  __shared__ double ldsd[kRowTile * 256];
+  int32_t *lds = reinterpret_cast<int32_t *>(ldsd);
   const int y = blockIdx.y;
-  const int p = L.prime[y];
-  const double q = P.qd[p], qr = P.qr[p];
+  const LimbView v = limb_view(L, y);
+  const double q = P.qd[v.p], qr = P.qr[v.p];
   const ModF m{q, qr};
+  const uint32_t *__restrict__ w = tw + (size_t)v.p * kN;
   const int rl = threadIdx.x >> 4;
   const int t16 = threadIdx.x & 15;
-  const int r = blockIdx.x * kTile + rl;
-  const uint32_t *__restrict__ src =
-      L.src[L.sel[y] & 3] + (size_t)L.src_limb[y] * kN + r * 256;
-  uint32_t *__restrict__ dst = L.dst[(L.sel[y] >> 2) & 3] + (size_t)L.dst_limb[y] * kN + r * 256;
-  const uint32_t *__restrict__ w = tw + (size_t)p * kN;
-  double *row = lds + rl * kLdsStride;
+  const int r = blockIdx.x * kRowTile + rl;
+  const size_t tile = (size_t)blockIdx.x * kRowTile * 256;
 
+  // phase 1 psi_inv_rev[2^(15-u) + r 2^(7-u) + t16 2^(3-u) + blk],
+  // phase 2 psi_inv_rev[2^(11-u) + r 2^(3-u) + blk]
+  uint32_t t1[15], t2[15];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int blk = 0; blk < (8 >> u); blk++) {
+      t1[16 - (16 >> u) + blk] =
+          w[(32768 >> u) + (r << (7 - u)) + (t16 << (3 - u)) + blk];
+      t2[16 - (16 >> u) + blk] = w[(2048 >> u) + (r << (3 - u)) + blk];
+    }
+  // phase 1 reads b = 16 t16 + j: 64 contiguous bytes per thread
   double x[16];
   {
-    const uint4 *v4 = reinterpret_cast<const uint4 *>(src + t16 * 16);
+    const uint4 *s4 =
+        reinterpret_cast<const uint4 *>(v.src + (size_t)r * 256 + t16 * 16);
+    if (L.flags & kInttLoadProduct) {
+      // Tensor's a1 (*) a2 (ElementWise.cu:504-546) fused into the load
+      const uint32_t qu = P.q[v.p];
+      const int32_t qinv = P.qinv[v.p];
+      const uint4 *t4 = reinterpret_cast<const uint4 *>(
+          L.src2[(L.sel[y] >> 4) & 3] + (size_t)L.src2_limb[y] * kN +
+          (size_t)r * 256 + t16 * 16);
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      uint4 v = v4[i];
-      x[4 * i + 0] = (double)v.x;
-      x[4 * i + 1] = (double)v.y;
-      x[4 * i + 2] = (double)v.z;
-      x[4 * i + 3] = (double)v.w;
+      for (int i = 0; i < 4; i++) {
+        const uint4 u = s4[i], w2 = t4[i];
+        x[4 * i + 0] = (double)mont_mul(u.x, w2.x, qu, qinv);
+        x[4 * i + 1] = (double)mont_mul(u.y, w2.y, qu, qinv);
+        x[4 * i + 2] = (double)mont_mul(u.z, w2.z, qu, qinv);
+        x[4 * i + 3] = (double)mont_mul(u.w, w2.w, qu, qinv);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const uint4 u = s4[i];
+        x[4 * i + 0] = (double)u.x;
+        x[4 * i + 1] = (double)u.y;
+        x[4 * i + 2] = (double)u.z;
+        x[4 * i + 3] = (double)u.w;
+      }
     }
   }
-  Tw16 t;
-  // phase 1: b = t16*16 + j, stages u = 0..3,
-  // twiddle psi_inv_rev[2^(15-u) + r*2^(7-u) + t16*2^(3-u) + blk]
+  gs16(x, t1, q, qr);
+  // GS sums double every stage: reduce before the next four stages
 #pragma unroll
-  for (int u = 0; u < 4; u++)
-#pragma unroll
-    for (int blk = 0; blk < (8 >> u); blk++)
-      set_tw(t, 16 - (16 >> u) + blk,
-             w[(32768 >> u) + (r << (7 - u)) + (t16 << (3 - u)) + blk], qr);
-  gs16(x, t, q);
-#pragma unroll
-  for (int j = 0; j < 16; j++) row[17 * t16 + j] = x[j];
+  for (int j = 0; j < 16; j++) ldsd[rlds(rl, 16 * t16 + j)] = reduce_f(x[j], m);
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < 16; k++) x[k] = row[t16 + 17 * k];
-  // phase 2: b = t16 + 16k, stages u = 4..7,
-  // twiddle psi_inv_rev[2^(11-u') + r*2^(3-u') + blk]
+  for (int k = 0; k < 16; k++) x[k] = ldsd[rlds(rl, t16 + 16 * k)];
+  gs16(x, t2, q, qr);
+  __syncthreads();  // int32 words alias doubles other threads still read
 #pragma unroll
-  for (int u = 0; u < 4; u++)
-#pragma unroll
-    for (int blk = 0; blk < (8 >> u); blk++)
-      set_tw(t, 16 - (16 >> u) + blk,
-             w[(2048 >> u) + (r << (3 - u)) + blk], qr);
-  gs16(x, t, q);
-#pragma unroll
-  for (int k = 0; k < 16; k++) dst[t16 + 16 * k] = canon_f(x[k], m);
+  for (int k = 0; k < 16; k++) lds[rlds(rl, t16 + 16 * k)] = lazy_i32(x[k], m);
+  __syncthreads();
+  lds_to_tile_rows(lds, v.dst + tile);
 }
 
 // ---------------------------------------------------------------------------
 // Inverse, column stages u = 8..15, then the per-limb constant
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void intt_cols(
+__global__ __launch_bounds__(kColThreads) void intt_cols(
     const NttLaunch L, const PrimeTab P, const uint32_t *__restrict__ tw) {
-  __shared__ double lds[256 * kColStride];
-  const int y = blockIdx.y;
-  const int p = L.prime[y];
-  const double q = P.qd[p], qr = P.qr[p];
+  __shared__ double ldsd[256 * kColTile];
+  int32_t *lds = reinterpret_cast<int32_t *>(ldsd);
+  const int y = blockIdx.x >> 3;  // same XCD-aware order as ntt_fwd_cols
+  const int c0 = (blockIdx.x & 7) * kColTile;
+  const LimbView v = limb_view(L, y);
+  const double q = P.qd[v.p], qr = P.qr[v.p];
   const ModF m{q, qr};
-  const uint32_t *__restrict__ src = L.src[L.sel[y] & 3] + (size_t)L.src_limb[y] * kN;
-  uint32_t *__restrict__ dst = L.dst[(L.sel[y] >> 2) & 3] + (size_t)L.dst_limb[y] * kN;
-  const uint32_t *__restrict__ w = tw + (size_t)p * kN;
-  const int cc = threadIdx.x & (kTile - 1);
-  const int rb = threadIdx.x >> 4;
-  const int c = blockIdx.x * kTile + cc;
+  const uint32_t *__restrict__ w = tw + (size_t)v.p * kN;
+  const int cc = threadIdx.x & 31;
+  const int rb = threadIdx.x >> 5;
 
+  // phase 1 psi_inv_rev[2^(7-u) + rb 2^(3-u) + blk],
+  // phase 2 psi_inv_rev[2^(3-u) + blk] (uniform)
+  uint32_t t1[15], t2[15];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int blk = 0; blk < (8 >> u); blk++) {
+      t1[16 - (16 >> u) + blk] = w[(128 >> u) + (rb << (3 - u)) + blk];
+      t2[16 - (16 >> u) + blk] = w[(8 >> u) + blk];
+    }
+  // the pass-1 intermediate is lazy signed int32
+  tile_cols_to_lds(v.src, c0, lds);
+  __syncthreads();
   double x[16];
 #pragma unroll
-  for (int j = 0; j < 16; j++) x[j] = (double)src[(rb * 16 + j) * 256 + c];
-  Tw16 t;
-  // phase 1: rows rb*16 + j, stages u = 8..11,
-  // twiddle psi_inv_rev[2^(7-u') + rb*2^(3-u') + blk]
+  for (int j = 0; j < 16; j++) x[j] = (double)lds[(rb * 16 + j) * 32 + cc];
+  gs16(x, t1, q, qr);
+  __syncthreads();  // staging words are reused by the double tile
 #pragma unroll
-  for (int u = 0; u < 4; u++)
-#pragma unroll
-    for (int blk = 0; blk < (8 >> u); blk++)
-      set_tw(t, 16 - (16 >> u) + blk, w[(128 >> u) + (rb << (3 - u)) + blk],
-             qr);
-  gs16(x, t, q);
-#pragma unroll
-  for (int j = 0; j < 16; j++) lds[(rb * 16 + j) * kColStride + cc] = x[j];
+  for (int j = 0; j < 16; j++)
+    ldsd[(rb * 16 + j) * 32 + cc] = reduce_f(x[j], m);
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < 16; k++) x[k] = lds[(rb + 16 * k) * kColStride + cc];
-  // phase 2: rows rb + 16k, stages u = 12..15, twiddle psi_inv_rev[2^(3-u') +
-  // blk]
-#pragma unroll
-  for (int u = 0; u < 4; u++)
-#pragma unroll
-    for (int blk = 0; blk < (8 >> u); blk++)
-      set_tw(t, 16 - (16 >> u) + blk, w[(8 >> u) + blk], qr);
-  gs16(x, t, q);
+  for (int k = 0; k < 16; k++) x[k] = ldsd[(rb + 16 * k) * 32 + cc];
+  gs16(x, t2, q, qr);
   const double cv = (double)L.c1[y], cq = cv * qr;
-  if (L.flags & kInttCentred) {
+  const bool centred = (L.flags & kInttCentred) != 0;
+  __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 16; k++)
-      dst[(rb + 16 * k) * 256 + c] =
-          (uint32_t)centre_f(mulmod_f(x[k], cv, cq, q), m);
-  } else {
-#pragma unroll
-    for (int k = 0; k < 16; k++)
-      dst[(rb + 16 * k) * 256 + c] = canon_f(mulmod_f(x[k], cv, cq, q), m);
+  for (int k = 0; k < 16; k++) {
+    const double r0 = reduce_f(mulmod_f(x[k], cv, cq, q), m);
+    // canonical [0, q) or centred [-(q-1)/2, (q-1)/2]
+    const double lo = centred ? -(q - 1.0) * 0.5 : 0.0;
+    double r1 = r0 < lo ? r0 + q : r0;
+    r1 = r1 >= lo + q ? r1 - q : r1;
+    lds[(rb + 16 * k) * 32 + cc] = (int32_t)r1;
   }
+  __syncthreads();
+  lds_to_tile_cols(lds, v.dst, c0);
 }
 
 // second pass reads what the first pass wrote
@@ -355,26 +502,24 @@ void in_place(NttLaunch &L) {
 // on dst (with the optional ModDown epilogue).
 void launch_ntt_fwd(const Context &ctx, const NttLaunch &L, hipStream_t st) {
   if (L.n <= 0) return;
-  dim3 grid(kTilesPerLimb, L.n);
-  hipLaunchKernelGGL(ntt_fwd_cols, grid, dim3(kThreads), 0, st, L, ctx.ptab,
-                     ctx.d_tw_fwd);
+  hipLaunchKernelGGL(ntt_fwd_cols, dim3(kColTiles * L.n), dim3(kColThreads), 0,
+                     st, L, ctx.ptab, ctx.d_tw_fwd);
   NttLaunch L2 = L;
   in_place(L2);
-  hipLaunchKernelGGL(ntt_fwd_rows, grid, dim3(kThreads), 0, st, L2, ctx.ptab,
-                     ctx.d_tw_fwd);
+  hipLaunchKernelGGL(ntt_fwd_rows, dim3(kRowTiles, L.n), dim3(kRowThreads), 0,
+                     st, L2, ctx.ptab, ctx.d_tw_fwd);
 }
 
 // Inverse NTT of L.n limbs: row stages into dst, then column stages in
 // place on dst with the post constant c1 (plain form) and optional centring.
 void launch_intt(const Context &ctx, const NttLaunch &L, hipStream_t st) {
   if (L.n <= 0) return;
-  dim3 grid(kTilesPerLimb, L.n);
-  hipLaunchKernelGGL(intt_rows, grid, dim3(kThreads), 0, st, L, ctx.ptab,
-                     ctx.d_tw_inv);
+  hipLaunchKernelGGL(intt_rows, dim3(kRowTiles, L.n), dim3(kRowThreads), 0, st,
+                     L, ctx.ptab, ctx.d_tw_inv);
   NttLaunch L2 = L;
   in_place(L2);
-  hipLaunchKernelGGL(intt_cols, grid, dim3(kThreads), 0, st, L2, ctx.ptab,
-                     ctx.d_tw_inv);
+  hipLaunchKernelGGL(intt_cols, dim3(kColTiles * L.n), dim3(kColThreads), 0, st,
+                     L2, ctx.ptab, ctx.d_tw_inv);
 }
 
 }  // namespace ck

@@ -7,6 +7,14 @@
 // Every launch plan is built on the host and passed by value; temporaries
 // come from the context workspace, so an operation issues no allocation, no
 // host<->device copy and no synchronisation.
+//
+// Fusions relative to the reference's kernel sequence (results identical):
+//   * BaseConv runs inside the forward NTT's column pass (ModUp and ModDown),
+//   * HMult's Tensor product a1 (*) a2 is formed while the ModUp INTT loads,
+//     and d0 = b1 (*) b2, d1 = b1 (*) a2 + a1 (*) b2 inside the key inner
+//     product's P * (d0, d1) fold,
+//   * ModUp's own-group limbs are read from the source by the inner product
+//     instead of being copied.
 #include "ck_ops.h"
 
 #include <algorithm>
@@ -22,12 +30,13 @@ int launch_status(const char *what) {
   return check_hip(hipGetLastError(), what);
 }
 
-// Split a list of limb jobs into launches of at most kMaxLaunchLimbs.
 struct NttJob {
   int prime, sel, src_limb, dst_limb, src2_limb;
   uint32_t c1, c2, c3;
+  int btab_off;
 };
 
+// Split a list of limb jobs into launches of at most kMaxLaunchLimbs.
 void run_jobs(const Context &c, bool forward, const NttLaunch &base,
               const std::vector<NttJob> &jobs, hipStream_t st) {
   for (size_t off = 0; off < jobs.size(); off += kMaxLaunchLimbs) {
@@ -40,6 +49,7 @@ void run_jobs(const Context &c, bool forward, const NttLaunch &base,
       L.src_limb[i] = (uint16_t)j.src_limb;
       L.dst_limb[i] = (uint16_t)j.dst_limb;
       L.src2_limb[i] = (int16_t)j.src2_limb;
+      L.btab_off[i] = (uint16_t)j.btab_off;
       L.c1[i] = j.c1;
       L.c2[i] = j.c2;
       L.c3[i] = j.c3;
@@ -77,7 +87,7 @@ int op_ntt(const Context &c, uint32_t *dst, const uint32_t *src, int nm,
     const int p = global_prime(c, nm, nt, t);
     // NTT(..., montgomery_conversion): x * R^2 * R^-1 = x * R
     const uint32_t cv = mont_conv ? h_to_mont(1, c.primes[p]) : 1u;
-    jobs.push_back({p, 0, t < num_q ? t : t + src_gap, t, -1, cv, 0, 0});
+    jobs.push_back({p, 0, t < num_q ? t : t + src_gap, t, -1, cv, 0, 0, 0});
   }
   run_jobs(c, true, L, jobs, st);
   return launch_status("ntt");
@@ -98,7 +108,7 @@ int op_intt_const(const Context &c, uint32_t *dst, const uint32_t *src,
     const int p = global_prime(c, nm, nt, t);
     const uint32_t q = c.primes[p];
     const uint32_t cv = h_mulmod(consts_mont[t] % q, r_inv_mod(q), q);
-    jobs.push_back({p, 0, t < num_q ? t : t + src_gap, t, -1, cv, 0, 0});
+    jobs.push_back({p, 0, t < num_q ? t : t + src_gap, t, -1, cv, 0, 0, 0});
   }
   run_jobs(c, false, L, jobs, st);
   return launch_status("intt");
@@ -121,62 +131,65 @@ int op_intt(const Context &c, uint32_t *dst, const uint32_t *src, int nm,
 // ---------------------------------------------------------------------------
 
 // ModUp (ModSwitch.cu:429-509). dst holds beta blocks of (num_q + alpha)
-// limbs; v is a workspace of num_q limbs.
+// limbs; v is a workspace of num_q limbs. When prod2 is given the source is
+// src (*) prod2 (HMult's a1 (*) a2). copy_own = false leaves each group's own
+// limbs unwritten (the fused inner product reads them from the source).
 int op_modup(const Context &c, int level, uint32_t *dst, const uint32_t *src,
-             uint32_t *v, hipStream_t st) {
+             uint32_t *v, hipStream_t st, bool copy_own,
+             const uint32_t *prod2) {
   const LevelPlan &lp = c.levels[level];
   const int tot = lp.num_q + c.alpha;
   // 1. INTT of all q limbs with the per-group constant, centred output
   {
     NttLaunch L = empty_launch();
     L.src[0] = src;
+    L.src2[0] = prod2;
     L.dst[0] = v;
-    L.flags = kInttCentred;
+    L.flags = kInttCentred | (prod2 ? kInttLoadProduct : 0);
     std::vector<NttJob> jobs;
     for (int t = 0; t < lp.num_q; t++)
-      jobs.push_back({lp.q_prime[t], 0, t, t, -1, lp.modup_c1[t], 0, 0});
+      jobs.push_back({lp.q_prime[t], 0, t, t, t, lp.modup_c1[t], 0, 0, 0});
     run_jobs(c, false, L, jobs, st);
   }
-  // 2. BaseConv of each group to every other limb, 3. copy own limbs
+  // 2. BaseConv of each group fused into the forward NTT of its outputs;
+  // up to four groups share a launch (one selector each)
+  NttLaunch L = empty_launch();
+  L.flags = kNttBconvInput;
+  std::vector<NttJob> jobs;
+  int nsel = 0;
+  auto flush = [&]() {
+    run_jobs(c, true, L, jobs, st);
+    jobs.clear();
+    nsel = 0;
+  };
   for (int g = 0; g < lp.beta; g++) {
     const ModUpGroup &gr = lp.groups[g];
-    uint32_t *out = dst + (size_t)g * tot * kLimb;
     if (gr.s1 <= gr.s0) continue;
-    BconvLaunch B;
-    std::memset(&B, 0, sizeof(B));
-    B.src[0] = reinterpret_cast<const int32_t *>(v + (size_t)gr.s0 * kLimb);
-    B.dst[0] = out;
-    B.table = gr.d_table;
-    B.s = gr.s1 - gr.s0;
-    B.d = (int)gr.dst_limb.size();
-    for (int m = 0; m < B.d; m++) {
-      B.dst_prime[m] = (uint8_t)gr.dst_prime[m];
-      B.dst_limb[m] = (uint16_t)gr.dst_limb[m];
+    const int s = gr.s1 - gr.s0;
+    const int d = (int)gr.dst_limb.size();
+    if (nsel == 4 || jobs.size() + d > (size_t)kMaxLaunchLimbs) flush();
+    const int sel = nsel++;
+    L.src[sel] = dst;
+    L.dst[sel] = dst;
+    L.bsrc[sel] = reinterpret_cast<const int32_t *>(v + (size_t)gr.s0 * kLimb);
+    L.btab[sel] = gr.d_ftab;
+    L.bs[sel] = s;
+    for (int m = 0; m < d; m++) {
+      const int limb = g * tot + gr.dst_limb[m];
+      jobs.push_back({gr.dst_prime[m], sel | (sel << 2) | (sel << 6), limb, limb,
+                      -1, 1, 0, 0, 2 * m * s});
     }
-    launch_bconv(c, B, 1, st);
-    int rc = check_hip(
-        hipMemcpyAsync(out + (size_t)gr.s0 * kLimb, src + (size_t)gr.s0 * kLimb,
-                       (size_t)(gr.s1 - gr.s0) * kLimb * sizeof(uint32_t),
-                       hipMemcpyDeviceToDevice, st),
-        "modup copy");
-    if (rc) return rc;
-  }
-  // 4. forward NTT of every converted limb of every group in one launch
-  {
-    NttLaunch L = empty_launch();
-    L.src[0] = dst;
-    L.dst[0] = dst;
-    std::vector<NttJob> jobs;
-    for (int g = 0; g < lp.beta; g++) {
-      const ModUpGroup &gr = lp.groups[g];
-      if (gr.s1 <= gr.s0) continue;
-      for (size_t m = 0; m < gr.dst_limb.size(); m++) {
-        const int limb = g * tot + gr.dst_limb[m];
-        jobs.push_back({gr.dst_prime[m], 0, limb, limb, -1, 1, 0, 0});
-      }
+    if (copy_own) {
+      int rc = check_hip(
+          hipMemcpyAsync(dst + ((size_t)g * tot + gr.s0) * kLimb,
+                         src + (size_t)gr.s0 * kLimb,
+                         (size_t)s * kLimb * sizeof(uint32_t),
+                         hipMemcpyDeviceToDevice, st),
+          "modup copy");
+      if (rc) return rc;
     }
-    run_jobs(c, true, L, jobs, st);
   }
+  if (!jobs.empty()) flush();
   return launch_status("modup");
 }
 
@@ -199,6 +212,10 @@ int op_moddown(const Context &c, int level, int kind, int npoly,
     set_error("no ModDown plan for this level and kind");
     return kErrInvalidArg;
   }
+  if ((size_t)npoly * mp->src_len * kLimb > c.ws_words) {
+    set_error("ModDown workspace too small");
+    return kErrUnsupported;
+  }
   // 1. INTT of the removed limbs, x const1, centred
   {
     NttLaunch L = empty_launch();
@@ -208,41 +225,29 @@ int op_moddown(const Context &c, int level, int kind, int npoly,
       L.dst[j] = v + (size_t)j * mp->src_len * kLimb;
       for (int i = 0; i < mp->src_len; i++)
         jobs.push_back({mp->src_prime[i], j | (j << 2), mp->src_limb[i], i, -1,
-                        mp->c1[i], 0, 0});
+                        mp->c1[i], 0, 0, 0});
     }
     L.flags = kInttCentred;
     run_jobs(c, false, L, jobs, st);
   }
-  // 2. BaseConv to the destination primes
-  {
-    BconvLaunch B;
-    std::memset(&B, 0, sizeof(B));
-    for (int j = 0; j < npoly; j++) {
-      B.src[j] =
-          reinterpret_cast<const int32_t *>(v + (size_t)j * mp->src_len * kLimb);
-      B.dst[j] = dst[j];
-    }
-    B.table = mp->d_table;
-    B.s = mp->src_len;
-    B.d = mp->dst_len;
-    for (int m = 0; m < B.d; m++) {
-      B.dst_prime[m] = (uint8_t)mp->dst_prime[m];
-      B.dst_limb[m] = (uint16_t)m;
-    }
-    launch_bconv(c, B, npoly, st);
-  }
-  // 3. forward NTT in place with the fused epilogue
+  // 2. BaseConv fused into the forward NTT, 3. epilogue in the row pass
   {
     NttLaunch L = empty_launch();
-    L.flags = kNttModDownEpilogue | (mp->use_pad ? kNttEpiloguePad : 0);
+    L.flags = kNttBconvInput | kNttModDownEpilogue |
+              (mp->use_pad ? kNttEpiloguePad : 0);
     std::vector<NttJob> jobs;
     for (int j = 0; j < npoly; j++) {
       L.src[j] = dst[j];
       L.dst[j] = dst[j];
       L.src2[j] = src[j];
+      L.bsrc[j] =
+          reinterpret_cast<const int32_t *>(v + (size_t)j * mp->src_len * kLimb);
+      L.btab[j] = mp->d_ftab;
+      L.bs[j] = mp->src_len;
       for (int y = 0; y < mp->dst_len; y++)
-        jobs.push_back({mp->dst_prime[y], j | (j << 2) | (j << 4), y, y,
-                        mp->src2_limb[y], 1, mp->pad[y], mp->inv_prod[y]});
+        jobs.push_back({mp->dst_prime[y], j | (j << 2) | (j << 4) | (j << 6), y,
+                        y, mp->src2_limb[y], 1, mp->pad[y], mp->inv_prod[y],
+                        2 * y * mp->src_len});
     }
     run_jobs(c, true, L, jobs, st);
   }
@@ -254,18 +259,20 @@ int op_moddown(const Context &c, int level, int kind, int npoly,
 // ---------------------------------------------------------------------------
 
 // MultKeyNoModDown (Context.cpp:609-712): ModUp(x), key inner product,
-// accum[q] += P * add. Workspace: v (num_q) + modup (beta * tot).
+// accum[q] += P * add. Workspace: v (num_q) + modup (beta * tot). With
+// `tensor` (b1, a1, b2, a2) the key-switched polynomial is a1 (*) a2 and
+// the fold adds P * (b1 b2, b1 a2 + a1 b2) (HMult).
 int op_mult_key_no_moddown(const Context &c, int level, uint32_t *acc_b,
                            uint32_t *acc_a, const uint32_t *x,
                            const uint32_t *add_b, const uint32_t *add_a,
                            const uint32_t *const *key_b,
                            const uint32_t *const *key_a, uint32_t *ws,
-                           hipStream_t st) {
+                           hipStream_t st, const uint32_t *const *tensor) {
   const LevelPlan &lp = c.levels[level];
-  const int tot = lp.num_q + c.alpha;
   uint32_t *v = ws;
   uint32_t *modup = ws + (size_t)lp.num_q * kLimb;
-  int rc = op_modup(c, level, modup, x, v, st);
+  int rc = op_modup(c, level, modup, tensor ? tensor[1] : x, v, st, false,
+                    tensor ? tensor[3] : nullptr);
   if (rc) return rc;
   KeyAccumArgs A;
   std::memset(&A, 0, sizeof(A));
@@ -274,16 +281,22 @@ int op_mult_key_no_moddown(const Context &c, int level, uint32_t *acc_b,
   for (int g = 0; g < lp.beta; g++) {
     if (lp.groups[g].s1 <= lp.groups[g].s0) continue;
     A.group[A.ngroups++] = g;
+    A.own0[g] = lp.groups[g].s0;
+    A.own1[g] = lp.groups[g].s1;
     A.key_b[g] = key_b[g];
     A.key_a[g] = key_a[g];
   }
+  A.x = x;
   A.add_b = add_b;
   A.add_a = add_a;
+  if (tensor) {
+    for (int i = 0; i < 4; i++) A.tensor[i] = tensor[i];
+    A.use_tensor = 1;
+  }
   A.p_prod = c.d_p_prod;
   A.out_b = acc_b;
   A.out_a = acc_a;
   EwNp np = make_np(c, lp.nm, lp.nt, c.alpha);
-  (void)tot;
   launch_key_accum(c, np, A, st);
   return launch_status("key accumulate");
 }
@@ -313,25 +326,24 @@ int op_relinearize(const Context &c, int level, uint32_t *out_b,
   return op_moddown(c, level, rescale ? 2 : 0, 2, dst, src, rest, st);
 }
 
-// HMult (Context.cpp:862-871): Tensor then Relinearize(Rescale)
+// HMult (Context.cpp:862-871): Tensor then Relinearize(Rescale), with the
+// Tensor folded into the ModUp INTT load and the P * (d0, d1) fold.
 int op_hmult(const Context &c, int level, uint32_t *out_b, uint32_t *out_a,
              const uint32_t *b1, const uint32_t *a1, const uint32_t *b2,
              const uint32_t *a2, const uint32_t *const *key_b,
              const uint32_t *const *key_a, bool rescale, hipStream_t st) {
   const LevelPlan &lp = c.levels[level];
-  uint32_t *d = c.ws;
-  const size_t qs = (size_t)lp.num_q * kLimb;
-  EwNp np = make_np(c, lp.nm, lp.nt, 0);
-  EwOperand x{{const_cast<uint32_t *>(b1), const_cast<uint32_t *>(a1), nullptr},
-              0};
-  EwOperand y{{const_cast<uint32_t *>(b2), const_cast<uint32_t *>(a2), nullptr},
-              0};
-  EwOperand o{{d, d + qs, d + 2 * qs}, 0};
-  launch_tensor(c, np, x, y, o, st);
-  int rc = launch_status("tensor");
+  const int tot = lp.num_q + c.alpha;
+  uint32_t *acc_b = c.ws;
+  uint32_t *acc_a = c.ws + (size_t)tot * kLimb;
+  uint32_t *rest = c.ws + 2 * (size_t)tot * kLimb;
+  const uint32_t *tensor[4] = {b1, a1, b2, a2};
+  int rc = op_mult_key_no_moddown(c, level, acc_b, acc_a, nullptr, nullptr,
+                                  nullptr, key_b, key_a, rest, st, tensor);
   if (rc) return rc;
-  return op_relinearize(c, level, out_b, out_a, d, d + qs, d + 2 * qs, key_b,
-                        key_a, rescale, d + 3 * qs, st);
+  uint32_t *dst[2] = {out_b, out_a};
+  const uint32_t *src[2] = {acc_b, acc_a};
+  return op_moddown(c, level, rescale ? 2 : 0, 2, dst, src, rest, st);
 }
 
 // MultKey on (b, a) (Context.cpp:584-607): key-switch a, accum.b += P b,

@@ -13,7 +13,8 @@ int op_intt_const(const Context &c, uint32_t *dst, const uint32_t *src,
                   int nm, int nt, int na, int src_gap,
                   const uint32_t *consts_mont, bool centred, hipStream_t st);
 int op_modup(const Context &c, int level, uint32_t *dst, const uint32_t *src,
-             uint32_t *v, hipStream_t st);
+             uint32_t *v, hipStream_t st, bool copy_own = true,
+             const uint32_t *prod2 = nullptr);
 const ModDownPlan *moddown_plan(const Context &c, int level, int kind);
 int op_moddown(const Context &c, int level, int kind, int npoly,
                uint32_t *const *dst, const uint32_t *const *src, uint32_t *v,
@@ -24,7 +25,8 @@ int op_mult_key_no_moddown(const Context &c, int level, uint32_t *acc_b,
                            const uint32_t *add_b, const uint32_t *add_a,
                            const uint32_t *const *key_b,
                            const uint32_t *const *key_a, uint32_t *ws,
-                           hipStream_t st);
+                           hipStream_t st,
+                           const uint32_t *const *tensor = nullptr);
 int op_relinearize(const Context &c, int level, uint32_t *out_b,
                    uint32_t *out_a, const uint32_t *d0, const uint32_t *d1,
                    const uint32_t *d2, const uint32_t *const *key_b,
