@@ -26,6 +26,7 @@ if PKG not in sys.path:
 HBM_PEAK_GBS = 8000.0
 LEVEL = 23
 PARAMS = "rns24_aux8.json"
+PROFILE_STEPS = 20
 
 
 def log(msg):
@@ -41,6 +42,58 @@ def uniform_poly(torch, ctx, pidx, gen):
         out[t] = torch.randint(0, q, (n,), dtype=torch.int64, device="cuda",
                                generator=gen).to(torch.int32)
     return out
+
+
+def alg_limbs(ctx, level, name):
+    """(limbs read, limbs written) that one launch of kernel `name` ("kernel/
+    output limbs") must move at least, for HMult+relin+rescale at `level`:
+    every operand read once and every result written once, 4 bytes a
+    coefficient."""
+    kern, n = name.rsplit("/", 1)
+    n = int(n)
+    num_q, alpha, beta = ctx.num_q(level), ctx.alpha, ctx.modup_beta(level)
+    next_q = ctx.num_q(level - 1)
+    if kern == "intt_rows+tensor":       # a1 (*) a2 -> INTT pass 1
+        return 2 * n, n
+    if kern in ("intt_rows", "intt_cols", "ntt_fwd_rows", "ntt_fwd_cols", "permute"):
+        return n, n
+    if kern == "ntt_fwd_cols+bconv":
+        if n == beta * (num_q + alpha) - num_q:   # ModUp: the q limbs, once
+            return num_q, n
+        polys = n // next_q                        # ModDown(+Rescale)
+        return polys * (alpha + num_q - next_q), n
+    if kern == "ntt_fwd_rows+epilogue":            # + the kept q limbs
+        return 2 * n, n
+    if kern == "tensor":
+        return 4 * num_q, 3 * num_q
+    if kern == "key_accum":                        # ModUp out + ct1, ct2 + key
+        ext = num_q + alpha
+        return (beta * ext - num_q) + 4 * num_q + 2 * beta * ext, 2 * ext
+    raise ValueError(f"no byte model for kernel {name}")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    path = os.path.join(REPO, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = rec.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("hbm_bytes")
+
+
+def copy_calibration(torch):
+    """One plain device-to-device copy of 256 MiB: 256 MiB read + 256 MiB
+    written, the reference point for scaling the HBM counters."""
+    src = torch.ones(64 << 20, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log("copy calibration: 268435456 bytes copied")
 
 
 def cpu_baseline(prm, seconds=10.0):
@@ -80,6 +133,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one plain 256 MiB device copy (HBM counter "
+                         "calibration for rocprofv3 --pmc runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported "
@@ -123,26 +179,34 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = e0.elapsed_time(e1) / a.steps
-    log(f"timed {a.steps} steps: {ms * 1e3:.1f} us/step (wall {wall * 1e3 / a.steps:.1f} us)")
+    log(f"timed {a.steps} steps: {ms * 1e3:.1f} us/step (host wall "
+        f"{wall * 1e6 / a.steps:.1f} us/step)")
 
-    # Dominant kernel family: the forward NTT (column + row passes), timed
-    # live on the same stream over the extended basis (32 limbs).
-    x = uniform_poly(torch, ctx, allp, gen)
-    y = torch.empty_like(x)
-    reps = 50
-    for _ in range(3):
-        ctx.ntt(y, x, ctx.max_main, ctx.max_ter, ctx.alpha, stream=stream)
+    # Per-kernel durations, measured live: HIP events recorded by the
+    # library on `stream` around every launch of a further run of steps.
+    ctx.profile(True)
+    for _ in range(PROFILE_STEPS):
+        step()
     torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(reps):
-        ctx.ntt(y, x, ctx.max_main, ctx.max_ter, ctx.alpha, stream=stream)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ntt_s = e0.elapsed_time(e1) / reps / 1e3
-    limbs = x.shape[0]
-    alg_bytes = limbs * ctx.n * 8  # read once + write once per coefficient
-    achieved = alg_bytes / ntt_s / 1e9
-    log(f"forward NTT of {limbs} limbs: {ntt_s * 1e6:.1f} us, {achieved:.0f} GB/s")
+    rep = ctx.profile_report()
+    ctx.profile(False)
+    per_step = {k: (v[0] * 1e3 / PROFILE_STEPS, v[1] // PROFILE_STEPS, v[0] * 1e3 / v[1])
+                for k, v in rep.items()}
+    total = sum(v[0] for v in per_step.values())
+    for k, (us, calls, avg) in sorted(per_step.items(), key=lambda kv: -kv[1][0]):
+        log(f"  {k:28s} {calls} x {avg:7.2f} us = {us:7.2f} us/step "
+            f"({100 * us / total:4.1f}%)")
+    log(f"  kernel sum {total:.1f} us/step")
+    dom = max(per_step, key=lambda k: per_step[k][0])
+    launch_us = per_step[dom][2]
+    limbs_in, limbs_out = alg_limbs(ctx, LEVEL, dom)
+    alg_bytes = (limbs_in + limbs_out) * ctx.n * 4
+    achieved = alg_bytes / (launch_us * 1e-6) / 1e9
+    log(f"dominant kernel {dom}: {launch_us:.2f} us/launch, {alg_bytes / 1e6:.1f} MB "
+        f"algorithmic -> {achieved:.0f} GB/s")
+    traffic = measured_traffic(dom)
+    if a.copy_calibration:
+        copy_calibration(torch)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -166,10 +230,10 @@ def main():
         "config": {"workload": "HMult+relin+rescale, N=2^16, L=24, alpha=8, dnum=3, "
                                "plain HIP form", "log_n": 16, "L": ctx.L, "alpha": ctx.alpha,
                    "dnum": ctx.dnum, "level": LEVEL, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "ntt_fwd_cols+ntt_fwd_rows (one forward NTT)",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "launch_us": ntt_s * 1e6, "alg_bytes_per_launch": alg_bytes},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "launch_us": launch_us,
+                     "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

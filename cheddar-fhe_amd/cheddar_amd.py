@@ -70,6 +70,10 @@ _SIGS = {
                                 ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
     "ck_rescale": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp),
                                   ctypes.POINTER(_vp), _vp]),
+    "ck_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "ck_profile_report": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p,
+                                         ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_longlong)]),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -278,6 +282,26 @@ class Context:
     def rescale(self, dsts, srcs, level, stream=None):
         _check(lib.ck_rescale(self._h, level, len(dsts), _ptrs(dsts), _ptrs(srcs),
                               _stream(stream)))
+
+    # ---- per-kernel timing --------------------------------------------------
+    def profile(self, on=True):
+        _check(lib.ck_profile_enable(self._h, int(on)))
+
+    def profile_report(self, max_entries=64):
+        """{kernel name: (total ms, launches)} of the launches recorded since
+        profile(True)"""
+        names = ctypes.create_string_buffer(64 * max_entries)
+        ms = (ctypes.c_double * max_entries)()
+        calls = (ctypes.c_longlong * max_entries)()
+        n = lib.ck_profile_report(self._h, max_entries, names, ms, calls)
+        if n < 0:
+            raise CheddarError(lib.ck_last_error().decode())
+        raw = names.raw
+        out = {}
+        for i in range(n):
+            name = raw[64 * i:64 * (i + 1)].split(b"\0", 1)[0].decode()
+            out[name] = (ms[i], calls[i])
+        return out
 
 
 # ---- host <-> device helpers ------------------------------------------------

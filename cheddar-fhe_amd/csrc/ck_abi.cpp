@@ -412,4 +412,31 @@ int ck_rescale(ck_context *ctx, int level, int npoly, uint32_t *const *dst,
   return ck_moddown(ctx, level, CK_RESCALE, npoly, dst, src, stream);
 }
 
+int ck_profile_enable(ck_context *ctx, int on) {
+  CK_REQUIRE(ctx && ctx->c, "null context");
+  prof_reset(*ctx->c);
+  ctx->c->prof_on = (on != 0);
+  return CK_OK;
+}
+
+int ck_profile_report(ck_context *ctx, int max_entries, char *names,
+                      double *total_ms, long long *calls) {
+  if (!ctx || !ctx->c || max_entries < 0 || (max_entries && (!names || !total_ms || !calls))) {
+    set_error("invalid profile report arguments");
+    return -1;
+  }
+  std::vector<std::string> nm;
+  std::vector<double> ms;
+  std::vector<long> cnt;
+  if (prof_report(*ctx->c, nm, ms, cnt)) return -1;
+  const int n = (int)nm.size() < max_entries ? (int)nm.size() : max_entries;
+  for (int i = 0; i < n; i++) {
+    std::strncpy(names + 64 * i, nm[i].c_str(), 63);
+    names[64 * i + 63] = 0;
+    total_ms[i] = ms[i];
+    calls[i] = cnt[i];
+  }
+  return n;
+}
+
 }  // extern "C"

@@ -457,10 +457,85 @@ int context_create(int log_n, const uint32_t *main_primes, int n_main,
   return kOk;
 }
 
+void prof_free(const Context &c);
+
 void context_destroy(Context *ctx) {
   if (!ctx) return;
+  prof_free(*ctx);
   for (void *p : ctx->allocations) (void)hipFree(p);
   delete ctx;
+}
+
+}  // namespace ck
+
+namespace ck {
+
+namespace {
+hipEvent_t pooled_event(const Context &c) {
+  if (!c.prof_pool.empty()) {
+    hipEvent_t e = c.prof_pool.back();
+    c.prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+}  // namespace
+
+int prof_begin(const Context &c, const char *kernel, int limbs, hipStream_t st) {
+  if (!c.prof_on) return -1;
+  const std::string key = std::string(kernel) + "/" + std::to_string(limbs);
+  int id = -1;
+  for (size_t i = 0; i < c.prof_names.size(); i++)
+    if (c.prof_names[i] == key) id = (int)i;
+  if (id < 0) {
+    id = (int)c.prof_names.size();
+    c.prof_names.push_back(key);
+  }
+  Context::ProfEvent ev{id, pooled_event(c), pooled_event(c)};
+  if (!ev.start || !ev.stop) return -1;
+  (void)hipEventRecord(ev.start, st);
+  c.prof_events.push_back(ev);
+  return (int)c.prof_events.size() - 1;
+}
+
+void prof_end(const Context &c, int token, hipStream_t st) {
+  if (token < 0) return;
+  (void)hipEventRecord(c.prof_events[token].stop, st);
+}
+
+// Accumulate the recorded launches per kernel name (waits for the events).
+int prof_report(const Context &c, std::vector<std::string> &names,
+                std::vector<double> &total_ms, std::vector<long> &calls) {
+  names = c.prof_names;
+  total_ms.assign(names.size(), 0.0);
+  calls.assign(names.size(), 0);
+  for (const auto &ev : c.prof_events) {
+    int rc = check_hip(hipEventSynchronize(ev.stop), "profile sync");
+    if (rc) return rc;
+    float ms = 0.f;
+    rc = check_hip(hipEventElapsedTime(&ms, ev.start, ev.stop), "profile time");
+    if (rc) return rc;
+    total_ms[ev.kernel] += ms;
+    calls[ev.kernel] += 1;
+  }
+  return kOk;
+}
+
+void prof_reset(const Context &c) {
+  for (const auto &ev : c.prof_events) {
+    c.prof_pool.push_back(ev.start);
+    c.prof_pool.push_back(ev.stop);
+  }
+  c.prof_events.clear();
+  c.prof_names.clear();
+}
+
+void prof_free(const Context &c) {
+  prof_reset(c);
+  for (hipEvent_t e : c.prof_pool) (void)hipEventDestroy(e);
+  c.prof_pool.clear();
 }
 
 }  // namespace ck

@@ -272,23 +272,29 @@ void launch_const(const Context &ctx, int op, const EwNp &np,
 void launch_tensor(const Context &ctx, const EwNp &np, const EwOperand &c1,
                    const EwOperand &c2, const EwOperand &out, hipStream_t st) {
   if (np.num_total <= 0) return;
+  const int t = prof_begin(ctx, "tensor", np.num_total, st);
   hipLaunchKernelGGL(tensor_kernel, ew_grid(np), dim3(kThreads), 0, st, np,
                      ctx.ptab, c1, c2, out);
+  prof_end(ctx, t, st);
 }
 
 void launch_key_accum(const Context &ctx, const EwNp &np,
                       const KeyAccumArgs &A, hipStream_t st) {
   if (np.num_total <= 0) return;
+  const int t = prof_begin(ctx, "key_accum", np.num_total, st);
   hipLaunchKernelGGL(key_accum_kernel, ew_grid(np), dim3(kThreads), 0, st, np,
                      ctx.ptab, A);
+  prof_end(ctx, t, st);
 }
 
 void launch_permute(const Context &ctx, const EwNp &np, const EwOperand &x,
                     const EwOperand &out, uint32_t g, int npoly,
                     hipStream_t st) {
   if (np.num_total <= 0) return;
+  const int t = prof_begin(ctx, "permute", np.num_total, st);
   hipLaunchKernelGGL(permute_kernel, dim3(65536 / kThreads, np.num_total),
                      dim3(kThreads), 0, st, np, x, out, g, npoly);
+  prof_end(ctx, t, st);
 }
 
 }  // namespace ck

@@ -133,7 +133,26 @@ struct Context {
   size_t ws_words = 0;
 
   std::vector<void *> allocations;
+
+  // optional per-kernel timing: HIP events recorded around every launch on
+  // the caller's stream while enabled (ck_profile_*)
+  struct ProfEvent {
+    int kernel;
+    hipEvent_t start, stop;
+  };
+  mutable bool prof_on = false;
+  mutable std::vector<ProfEvent> prof_events;
+  mutable std::vector<hipEvent_t> prof_pool;
+  mutable std::vector<std::string> prof_names;
 };
+
+// Per-kernel timing hooks: prof_begin returns a token (-1 when profiling is
+// off) that prof_end closes; both only record events on the stream.
+int prof_begin(const Context &c, const char *kernel, int limbs, hipStream_t st);
+void prof_end(const Context &c, int token, hipStream_t st);
+int prof_report(const Context &c, std::vector<std::string> &names,
+                std::vector<double> &total_ms, std::vector<long> &calls);
+void prof_reset(const Context &c);
 
 // ---- context.cpp ----
 int context_create(int log_n, const uint32_t *main_primes, int n_main,

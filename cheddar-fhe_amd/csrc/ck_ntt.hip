@@ -502,24 +502,35 @@ void in_place(NttLaunch &L) {
 // on dst (with the optional ModDown epilogue).
 void launch_ntt_fwd(const Context &ctx, const NttLaunch &L, hipStream_t st) {
   if (L.n <= 0) return;
+  int t = prof_begin(ctx, (L.flags & kNttBconvInput) ? "ntt_fwd_cols+bconv"
+                                                     : "ntt_fwd_cols", L.n, st);
   hipLaunchKernelGGL(ntt_fwd_cols, dim3(kColTiles * L.n), dim3(kColThreads), 0,
                      st, L, ctx.ptab, ctx.d_tw_fwd);
+  prof_end(ctx, t, st);
   NttLaunch L2 = L;
   in_place(L2);
+  t = prof_begin(ctx, (L.flags & kNttModDownEpilogue) ? "ntt_fwd_rows+epilogue"
+                                                      : "ntt_fwd_rows", L.n, st);
   hipLaunchKernelGGL(ntt_fwd_rows, dim3(kRowTiles, L.n), dim3(kRowThreads), 0,
                      st, L2, ctx.ptab, ctx.d_tw_fwd);
+  prof_end(ctx, t, st);
 }
 
 // Inverse NTT of L.n limbs: row stages into dst, then column stages in
 // place on dst with the post constant c1 (plain form) and optional centring.
 void launch_intt(const Context &ctx, const NttLaunch &L, hipStream_t st) {
   if (L.n <= 0) return;
+  int t = prof_begin(ctx, (L.flags & kInttLoadProduct) ? "intt_rows+tensor"
+                                                       : "intt_rows", L.n, st);
   hipLaunchKernelGGL(intt_rows, dim3(kRowTiles, L.n), dim3(kRowThreads), 0, st,
                      L, ctx.ptab, ctx.d_tw_inv);
+  prof_end(ctx, t, st);
   NttLaunch L2 = L;
   in_place(L2);
+  t = prof_begin(ctx, "intt_cols", L.n, st);
   hipLaunchKernelGGL(intt_cols, dim3(kColTiles * L.n), dim3(kColThreads), 0, st,
                      L2, ctx.ptab, ctx.d_tw_inv);
+  prof_end(ctx, t, st);
 }
 
 }  // namespace ck

@@ -206,6 +206,18 @@ int ck_hconj(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
 int ck_rescale(ck_context *ctx, int level, int npoly, uint32_t *const *dst,
                const uint32_t *const *src, void *stream);
 
+/* ---- Per-kernel timing ---------------------------------------------------- */
+/* Replaces the wall-clock profiling macros of the reference's test harness
+ * (unittest/Testbed.h:24-47) with device-side timing: while enabled, every
+ * kernel the context launches is bracketed by HIP events on the caller's
+ * stream (enabling or disabling clears earlier records). */
+int ck_profile_enable(ck_context *ctx, int on);
+/* Waits for the recorded events and writes, per kernel name, the summed
+ * duration and the launch count; names is max_entries x 64 bytes. Returns
+ * the number of entries (or -1 on error). */
+int ck_profile_report(ck_context *ctx, int max_entries, char *names,
+                      double *total_ms, long long *calls);
+
 #ifdef __cplusplus
 }
 #endif

@@ -113,6 +113,8 @@ class Param:
     """Oracle parameter set: prime chain, level table and per-prime tables."""
 
     def __init__(self, log_n, main_primes, aux_primes, ter_primes=(), level_config=None):
+        if any(int(q) >= 2 ** 31 for q in list(main_primes) + list(aux_primes) + list(ter_primes)):
+            raise ValueError("primes must be below 2^31 (32-bit words)")
         main = np.ascontiguousarray(main_primes, dtype=np.uint32)
         ter = np.ascontiguousarray(ter_primes, dtype=np.uint32)
         aux = np.ascontiguousarray(aux_primes, dtype=np.uint32)
