@@ -72,6 +72,13 @@ def alg_limbs(ctx, level, name):
     raise ValueError(f"no byte model for kernel {name}")
 
 
+def kernel_key(name):
+    """'ntt_fwd_cols+bconv/72' -> 'ntt_fwd_cols/72': the kernel symbol and
+    its limb count, the key rocprof summaries use."""
+    kern, n = name.rsplit("/", 1)
+    return f"{kern.split('+')[0]}/{n}"
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
@@ -81,8 +88,92 @@ def measured_traffic(kernel):
             rec = json.load(f)
     except (OSError, ValueError):
         return None
-    k = rec.get("kernels", {}).get(kernel)
+    k = rec.get("kernels", {}).get(kernel_key(kernel))
     return None if k is None else k.get("hbm_bytes")
+
+
+# ---- rocprofv3 output summaries --------------------------------------------
+# Launch grids -> limb counts: column passes run 8 workgroups per limb,
+# row passes and element-wise kernels 16 / 64 per limb (grid y = limbs).
+def _short(sym):
+    s = sym.replace("void ", "").replace("ck::(anonymous namespace)::", "")
+    s = s.replace("ck::", "").split("(")[0]
+    s = s.split("<")[0]
+    return s[:-7] if s.endswith("_kernel") else s
+
+
+def _limbs(short, grid_x, grid_y, wg_x):
+    wgs = grid_x // max(1, wg_x)
+    if short in ("ntt_fwd_cols", "intt_cols"):
+        return wgs // 8
+    if grid_y > 1:
+        return grid_y
+    return None
+
+
+def _read_csv(path):
+    import csv
+    with open(path) as f:
+        return list(csv.DictReader(f))
+
+
+def rocprof_summary(trace_csv):
+    """Markdown table of a --kernel-trace CSV: per library kernel and limb
+    count, launches and mean/median duration."""
+    groups = {}
+    for r in _read_csv(trace_csv):
+        short = _short(r["Kernel_Name"])
+        n = _limbs(short, int(r["Grid_Size_X"]), int(r["Grid_Size_Y"]),
+                   int(r["Workgroup_Size_X"]))
+        if n is None:
+            continue
+        dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        groups.setdefault(f"{short}/{n}", []).append(dur)
+    lines = ["| kernel/limbs | launches | mean us | median us | min us |",
+             "|---|---|---|---|---|"]
+    for k, v in sorted(groups.items(), key=lambda kv: -sum(kv[1])):
+        v.sort()
+        lines.append(f"| {k} | {len(v)} | {sum(v) / len(v):.2f} | {v[len(v) // 2]:.2f} "
+                     f"| {v[0]:.2f} |")
+    return "\n".join(lines)
+
+
+def pmc_traffic(fetch_csv, write_csv):
+    """Per kernel/limbs HBM bytes per launch from FETCH_SIZE and WRITE_SIZE
+    (KiB per dispatch), each scaled so that the 256 MiB device copy of
+    --copy-calibration reads 256 MiB and writes 256 MiB."""
+    def per_kernel(path, counter):
+        acc, copy = {}, None
+        for r in _read_csv(path):
+            if r["Counter_Name"] != counter:
+                continue
+            v = float(r["Counter_Value"])
+            if "copyBuffer" in r["Kernel_Name"] and int(r["Grid_Size"]) == 131072:
+                copy = v
+                continue
+            short = _short(r["Kernel_Name"])
+            wg = int(r["Workgroup_Size"])
+            grid = int(r["Grid_Size"])
+            if short in ("ntt_fwd_cols", "intt_cols"):
+                n = grid // wg // 8
+            elif short in ("ntt_fwd_rows", "intt_rows"):
+                n = grid // wg // 16
+            elif short in ("key_accum", "tensor", "permute"):
+                n = grid // wg // 64
+            else:
+                continue
+            acc.setdefault(f"{short}/{n}", []).append(v)
+        return {k: sum(v) / len(v) for k, v in acc.items()}, copy
+    fetch, fcopy = per_kernel(fetch_csv, "FETCH_SIZE")
+    write, wcopy = per_kernel(write_csv, "WRITE_SIZE")
+    if not fcopy or not wcopy:
+        raise SystemExit("no 256 MiB calibration copy in the counter files")
+    fs, ws = 262144.0 / fcopy, 262144.0 / wcopy
+    out = {"fetch_scale": fs, "write_scale": ws, "kernels": {}}
+    for k in sorted(set(fetch) & set(write)):
+        out["kernels"][k] = {"fetch_kib": fetch[k], "write_kib": write[k],
+                             "hbm_bytes": int((fetch[k] * fs + write[k] * ws) * 1024)}
+    return out
 
 
 def copy_calibration(torch):
@@ -136,7 +227,19 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 256 MiB device copy (HBM counter "
                          "calibration for rocprofv3 --pmc runs)")
+    ap.add_argument("--rocprof-summary", metavar="KERNEL_TRACE_CSV",
+                    help="print a per-kernel summary of a rocprofv3 --kernel-trace "
+                         "CSV and exit")
+    ap.add_argument("--pmc-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="print per-kernel HBM traffic (JSON) from two rocprofv3 --pmc "
+                         "runs (FETCH_SIZE, WRITE_SIZE) and exit")
     a = ap.parse_args()
+    if a.rocprof_summary:
+        print(rocprof_summary(a.rocprof_summary))
+        return
+    if a.pmc_traffic:
+        print(json.dumps(pmc_traffic(*a.pmc_traffic), indent=1))
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported "
               "(the key-switching path has no multi-GPU form here)", file=sys.stderr)

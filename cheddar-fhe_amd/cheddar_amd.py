@@ -70,6 +70,10 @@ _SIGS = {
                                 ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
     "ck_rescale": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp),
                                   ctypes.POINTER(_vp), _vp]),
+    "ck_hrot_hoisted": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _ip,
+                                       ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, _vp,
+                                       ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int,
+                                       _vp]),
     "ck_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ck_profile_report": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p,
                                          ctypes.POINTER(ctypes.c_double),
@@ -83,6 +87,15 @@ def load_library(path=LIB_PATH):
     """Load libcheddar_amd.so and declare every entry point's signature."""
     if not os.path.exists(path):
         raise ImportError(f"{path} is not built: run python cheddar-fhe_amd/build.py")
+    # One HIP runtime per process: torch ships its own libamdhip64 (SONAME
+    # libamdhip64.so.7, loaded by the name libamdhip64.so). Loading torch
+    # first lets this library's libamdhip64.so.7 dependency bind to that
+    # copy; the other order maps a second runtime and torch then finds no
+    # device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     for name, (res, args) in _SIGS.items():
         fn = getattr(lib, name)
@@ -282,6 +295,18 @@ class Context:
     def rescale(self, dsts, srcs, level, stream=None):
         _check(lib.ck_rescale(self._h, level, len(dsts), _ptrs(dsts), _ptrs(srcs),
                               _stream(stream)))
+
+    def hrot_hoisted(self, outs, ct, keys, rots, level, moddown=True, stream=None):
+        """Hoisted rotations of ct = (b, a) by each of rots (in [1, N/2)):
+        outs[i] = (b_i, a_i) receives HRot(ct, rots[i]) (moddown) or the
+        Q*P-basis baby-step output; keys[i] = (key_b blocks, key_a blocks)."""
+        n = len(rots)
+        kb = [t for k in keys for t in k[0]]
+        ka = [t for k in keys for t in k[1]]
+        _check(lib.ck_hrot_hoisted(self._h, level, n, (ctypes.c_int * n)(*rots),
+                                   _ptrs([o[0] for o in outs]), _ptrs([o[1] for o in outs]),
+                                   _ptr(ct[0]), _ptr(ct[1]), _ptrs(kb), _ptrs(ka),
+                                   int(moddown), _stream(stream)))
 
     # ---- per-kernel timing --------------------------------------------------
     def profile(self, on=True):

@@ -171,8 +171,9 @@ int ck_ntt(ck_context *ctx, uint32_t *dst, const uint32_t *src, int num_main,
   CK_REQUIRE(valid_np(*ctx->c, num_main, num_ter, num_aux), "invalid NP");
   CK_REQUIRE(src_gap >= 0 && (src_gap == 0 || src != dst),
              "a limb gap requires dst != src");
+  ImmediateSink sink(*ctx->c, as_stream(stream));
   return op_ntt(*ctx->c, dst, src, num_main, num_ter, num_aux, src_gap,
-                montgomery_conversion != 0, as_stream(stream));
+                montgomery_conversion != 0, sink);
 }
 
 int ck_intt(ck_context *ctx, uint32_t *dst, const uint32_t *src, int num_main,
@@ -182,8 +183,9 @@ int ck_intt(ck_context *ctx, uint32_t *dst, const uint32_t *src, int num_main,
   CK_REQUIRE(valid_np(*ctx->c, num_main, num_ter, num_aux), "invalid NP");
   CK_REQUIRE(src_gap >= 0 && (src_gap == 0 || src != dst),
              "a limb gap requires dst != src");
+  ImmediateSink sink(*ctx->c, as_stream(stream));
   return op_intt(*ctx->c, dst, src, num_main, num_ter, num_aux, src_gap,
-                 montgomery_conversion != 0, as_stream(stream));
+                 montgomery_conversion != 0, sink);
 }
 
 int ck_intt_mult_const(ck_context *ctx, uint32_t *dst, const uint32_t *src,
@@ -193,8 +195,9 @@ int ck_intt_mult_const(ck_context *ctx, uint32_t *dst, const uint32_t *src,
   CK_REQUIRE(valid_np(*ctx->c, num_main, num_ter, num_aux), "invalid NP");
   CK_REQUIRE(src_gap >= 0 && (src_gap == 0 || src != dst),
              "a limb gap requires dst != src");
+  ImmediateSink sink(*ctx->c, as_stream(stream));
   return op_intt_const(*ctx->c, dst, src, num_main, num_ter, num_aux, src_gap,
-                       consts, normalize != 0, as_stream(stream));
+                       consts, normalize != 0, sink);
 }
 
 int ck_modup(ck_context *ctx, int level, uint32_t *dst, const uint32_t *src,
@@ -203,7 +206,8 @@ int ck_modup(ck_context *ctx, int level, uint32_t *dst, const uint32_t *src,
   const Context &c = *ctx->c;
   CK_REQUIRE(valid_level(c, level), "invalid level");
   CK_REQUIRE(dst != src, "ModUp is not in place");
-  return op_modup(c, level, dst, src, c.ws, as_stream(stream));
+  ImmediateSink sink(c, as_stream(stream));
+  return op_modup(c, level, dst, src, c.ws, sink);
 }
 
 int ck_moddown(ck_context *ctx, int level, int kind, int npoly,
@@ -219,7 +223,8 @@ int ck_moddown(ck_context *ctx, int level, int kind, int npoly,
     for (int k = 0; k < npoly; k++)
       CK_REQUIRE(dst[j] != src[k], "ModDown dst and src must differ");
   CK_REQUIRE(moddown_plan(c, level, kind), "no such rescale at this level");
-  return op_moddown(c, level, kind, npoly, dst, src, c.ws, as_stream(stream));
+  ImmediateSink sink(c, as_stream(stream));
+  return op_moddown(c, level, kind, npoly, dst, src, c.ws, sink);
 }
 
 int ck_add(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly,
@@ -320,8 +325,9 @@ int ck_mult_key_no_moddown(ck_context *ctx, int level, uint32_t *accum_b,
              "null key block");
   CK_REQUIRE(accum_b != x && accum_a != x,
              "in-place operation is not supported for MultKeyNoModDown");
+  ImmediateSink sink(c, as_stream(stream));
   return op_mult_key_no_moddown(c, level, accum_b, accum_a, x, add_b, add_a,
-                                key_b, key_a, c.ws, as_stream(stream));
+                                key_b, key_a, c.ws, sink);
 }
 
 int ck_mult_key(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
@@ -333,8 +339,8 @@ int ck_mult_key(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
   CK_REQUIRE(valid_level(c, level), "invalid level");
   CK_REQUIRE(valid_key(c, key_b, level) && valid_key(c, key_a, level),
              "null key block");
-  return op_mult_key(c, level, out_b, out_a, b, a, key_b, key_a, c.ws,
-                     as_stream(stream));
+  ImmediateSink sink(c, as_stream(stream));
+  return op_mult_key(c, level, out_b, out_a, b, a, key_b, key_a, c.ws, sink);
 }
 
 int ck_relinearize(ck_context *ctx, int level, uint32_t *out_b,
@@ -349,8 +355,9 @@ int ck_relinearize(ck_context *ctx, int level, uint32_t *out_b,
              "cannot rescale at this level");
   CK_REQUIRE(valid_key(c, key_b, level) && valid_key(c, key_a, level),
              "null key block");
+  ImmediateSink sink(c, as_stream(stream));
   return op_relinearize(c, level, out_b, out_a, d0, d1, d2, key_b, key_a,
-                        rescale != 0, c.ws, as_stream(stream));
+                        rescale != 0, c.ws, sink);
 }
 
 int ck_hmult(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
@@ -365,8 +372,9 @@ int ck_hmult(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
              "cannot rescale at this level");
   CK_REQUIRE(valid_key(c, key_b, level) && valid_key(c, key_a, level),
              "null key block");
+  ImmediateSink sink(c, as_stream(stream));
   return op_hmult(c, level, out_b, out_a, b1, a1, b2, a2, key_b, key_a,
-                  rescale != 0, as_stream(stream));
+                  rescale != 0, c.ws, sink);
 }
 
 int ck_hrot(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
@@ -391,8 +399,9 @@ int ck_hrot(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
   }
   CK_REQUIRE(valid_key(c, key_b, level) && valid_key(c, key_a, level),
              "null key block");
+  ImmediateSink sink(c, as_stream(stream));
   return op_hrot_galois(c, level, out_b, out_a, b, a, key_b, key_a,
-                        c.galois[rot], as_stream(stream));
+                        c.galois[rot], c.ws, sink);
 }
 
 int ck_hconj(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
@@ -403,13 +412,42 @@ int ck_hconj(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
   CK_REQUIRE(valid_level(c, level), "invalid level");
   CK_REQUIRE(valid_key(c, key_b, level) && valid_key(c, key_a, level),
              "null key block");
+  ImmediateSink sink(c, as_stream(stream));
   return op_hrot_galois(c, level, out_b, out_a, b, a, key_b, key_a,
-                        2u * c.n - 1, as_stream(stream));
+                        2u * c.n - 1, c.ws, sink);
 }
 
 int ck_rescale(ck_context *ctx, int level, int npoly, uint32_t *const *dst,
                const uint32_t *const *src, void *stream) {
   return ck_moddown(ctx, level, CK_RESCALE, npoly, dst, src, stream);
+}
+
+int ck_hrot_hoisted(ck_context *ctx, int level, int n_rot, const int *rots,
+                    uint32_t *const *out_b, uint32_t *const *out_a,
+                    const uint32_t *b, const uint32_t *a,
+                    const uint32_t *const *key_b, const uint32_t *const *key_a,
+                    int moddown, void *stream) {
+  CK_REQUIRE(ctx && ctx->c && b && a && rots, "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_level(c, level), "invalid level");
+  CK_REQUIRE(n_rot >= 1, "n_rot must be positive");
+  CK_REQUIRE(valid_ptrs(out_b, n_rot) && valid_ptrs(out_a, n_rot),
+             "null output pointer");
+  CK_REQUIRE(key_b && key_a, "null key table");
+  for (int i = 0; i < n_rot; i++) {
+    CK_REQUIRE(rots[i] >= 1 && rots[i] < c.n / 2,
+               "hoisted rotation amounts must be in [1, N/2)");
+    CK_REQUIRE(valid_key(c, key_b + (size_t)i * c.dnum, level) &&
+                   valid_key(c, key_a + (size_t)i * c.dnum, level),
+               "null key block");
+    CK_REQUIRE(out_b[i] != b && out_b[i] != a && out_a[i] != b && out_a[i] != a,
+               "in-place operation is not supported for hoisted rotations");
+  }
+  CK_REQUIRE(hoisted_ws_limbs(c, level, n_rot, moddown != 0) * kLimb <= c.ws_words,
+             "workspace too small");
+  ImmediateSink sink(c, as_stream(stream));
+  return op_hoisted_rotations(c, level, n_rot, rots, out_b, out_a, b, a, key_b,
+                              key_a, moddown != 0, c.ws, sink);
 }
 
 int ck_profile_enable(ck_context *ctx, int on) {

@@ -17,6 +17,7 @@
 #include <mutex>
 
 #include "ck_internal.h"
+#include "ck_ops.h"
 
 namespace ck {
 
@@ -433,17 +434,13 @@ int context_create(int log_n, const uint32_t *main_primes, int n_main,
       return rc;
     }
   }
-  // workspace for the largest level: tensor (3 L) + ModUp INTT (L) +
-  // ModUp output (beta (L + alpha)) + accumulators (2 (L + alpha)) +
-  // ModDown INTT of two polys (2 (alpha + L))
+  // workspace for the largest operation at any level: HRot (its MultKey
+  // output + a relinearization's temporaries) or up to kMaxBsRot hoisted
+  // rotations with their ModDown
   size_t limbs = 0;
   for (int l = 0; l < n_levels; l++) {
-    const LevelPlan &lp = c->levels[l];
-    size_t need = 3 * (size_t)lp.num_q + lp.num_q +
-                  (size_t)lp.beta * (lp.num_q + c->alpha) +
-                  2 * (size_t)(lp.num_q + c->alpha) +
-                  2 * (size_t)(c->alpha + lp.num_q);
-    limbs = std::max(limbs, need);
+    const size_t hrot = 2 * (size_t)c->levels[l].num_q + relin_ws_limbs(*c, l);
+    limbs = std::max({limbs, hrot, hoisted_ws_limbs(*c, l, kMaxBsRot, true)});
   }
   c->ws_words = limbs * (size_t)n;
   rc = check_hip(hipMalloc((void **)&c->ws, c->ws_words * sizeof(uint32_t)),
@@ -483,8 +480,9 @@ hipEvent_t pooled_event(const Context &c) {
 }
 }  // namespace
 
-int prof_begin(const Context &c, const char *kernel, int limbs, hipStream_t st) {
-  if (!c.prof_on) return -1;
+bool prof_events(const Context &c, const char *kernel, int limbs,
+                 hipEvent_t *start, hipEvent_t *stop) {
+  if (!c.prof_on) return false;
   const std::string key = std::string(kernel) + "/" + std::to_string(limbs);
   int id = -1;
   for (size_t i = 0; i < c.prof_names.size(); i++)
@@ -494,15 +492,11 @@ int prof_begin(const Context &c, const char *kernel, int limbs, hipStream_t st) 
     c.prof_names.push_back(key);
   }
   Context::ProfEvent ev{id, pooled_event(c), pooled_event(c)};
-  if (!ev.start || !ev.stop) return -1;
-  (void)hipEventRecord(ev.start, st);
+  if (!ev.start || !ev.stop) return false;
   c.prof_events.push_back(ev);
-  return (int)c.prof_events.size() - 1;
-}
-
-void prof_end(const Context &c, int token, hipStream_t st) {
-  if (token < 0) return;
-  (void)hipEventRecord(c.prof_events[token].stop, st);
+  *start = ev.start;
+  *stop = ev.stop;
+  return true;
 }
 
 // Accumulate the recorded launches per kernel name (waits for the events).

@@ -48,6 +48,47 @@ struct KeyAccumArgs {
   uint32_t *out_a;
 };
 
+// Hoisted baby step (HoistHandler::BSFusedKeyMult + BSFusedKernel,
+// src/extension/Hoist.cu:21-78, :207-327) for up to kMaxBsRot rotations of
+// one ModUp'd polynomial:
+//   acc_r[y][x] = sum_g modup_g[y][x] (*) key_{r,g}[prime(y)][x]
+//                 (+ P (*) b[y][x] on q limbs)
+// stored at the automorphism's position (scatter by g_r^-1 = 5^(N/2 - r)).
+constexpr int kMaxBsRot = 8;
+constexpr int kMaxBsGroups = 8;
+
+struct BsFusedArgs {
+  const uint32_t *modup;  // beta blocks of num_total limbs
+  const uint32_t *x;      // ModUp source (own limbs are read from here)
+  const uint32_t *b;      // pseudo-ModUp input (num_q limbs)
+  const uint32_t *p_prod; // by global prime
+  int ngroups;
+  int nrot;
+  int group[kMaxBsGroups];
+  int own0[kMaxBsGroups];
+  int own1[kMaxBsGroups];
+  const uint32_t *key_b[kMaxBsRot][kMaxBsGroups];
+  const uint32_t *key_a[kMaxBsRot][kMaxBsGroups];
+  uint32_t *out_b[kMaxBsRot];  // num_total limbs each
+  uint32_t *out_a[kMaxBsRot];
+  uint32_t ginv[kMaxBsRot];
+};
+
+// Batched (table) forms: item i of a plan stage has its own layout and
+// arguments; job j covers limb jobs[j].y of item jobs[j].item.
+struct EwJob {
+  uint16_t item;
+  uint16_t y;
+};
+struct KeyAccumItem {
+  EwNp np;
+  KeyAccumArgs A;
+};
+struct BsFusedItem {
+  EwNp np;
+  BsFusedArgs A;
+};
+
 void launch_binary(const Context &ctx, int op, const EwNp &np,
                    const EwOperand &x, const EwOperand &y, const EwOperand &out,
                    int npoly, hipStream_t st);
@@ -64,6 +105,12 @@ void launch_key_accum(const Context &ctx, const EwNp &np,
 void launch_permute(const Context &ctx, const EwNp &np, const EwOperand &x,
                     const EwOperand &out, uint32_t g, int npoly,
                     hipStream_t st);
+void launch_bs_fused(const Context &ctx, const EwNp &np, const BsFusedArgs &A,
+                     hipStream_t st);
+void launch_key_accum_table(const Context &ctx, const KeyAccumItem *items,
+                            const EwJob *jobs, int njobs, hipStream_t st);
+void launch_bs_fused_table(const Context &ctx, const BsFusedItem *items,
+                           const EwJob *jobs, int njobs, hipStream_t st);
 
 inline EwNp make_np(const Context &c, int num_main, int num_ter, int num_aux) {
   EwNp np;

@@ -135,9 +135,13 @@ __global__ __launch_bounds__(kThreads) void tensor_kernel(
 // ElementWise::PAccum, CPAccum<2,false>) fused with the P * (b, a) fold
 // (CAccum, CPAccumAdd<2,true>):
 //   acc_x[y] = sum_g modup_g[y] (*) key_x_g[prime(y)]  (+ P (*) add_x[y])
-__global__ __launch_bounds__(kThreads) void key_accum_kernel(
-    const EwNp np, const PrimeTab P, const KeyAccumArgs A) {
-  EW_PROLOGUE
+__device__ __forceinline__ void key_accum_body(const EwNp &np,
+                                               const PrimeTab &P,
+                                               const KeyAccumArgs &A, int y,
+                                               int k) {
+  const int p = ew_prime(np, y);
+  const uint32_t q = P.q[p];
+  const int32_t qinv = P.qinv[p];
   const size_t tot_stride = (size_t)np.num_total * 65536;
   uint32_t rb[4] = {0, 0, 0, 0}, ra[4] = {0, 0, 0, 0};
   const size_t my = (size_t)y * 65536 + k;
@@ -208,6 +212,121 @@ __global__ __launch_bounds__(kThreads) void key_accum_kernel(
   st4(A.out_a + my, make_uint4(ra[0], ra[1], ra[2], ra[3]));
 }
 
+__global__ __launch_bounds__(kThreads) void key_accum_kernel(
+    const EwNp np, const PrimeTab P, const KeyAccumArgs A) {
+  key_accum_body(np, P, A, blockIdx.y, (blockIdx.x * kThreads + threadIdx.x) * kVec);
+}
+
+__global__ __launch_bounds__(kThreads) void key_accum_table_kernel(
+    const KeyAccumItem *__restrict__ items, const EwJob *__restrict__ jobs,
+    const PrimeTab P) {
+  const EwJob jb = jobs[blockIdx.y];
+  const KeyAccumItem &it = items[jb.item];
+  key_accum_body(it.np, P, it.A, jb.y,
+                 (blockIdx.x * kThreads + threadIdx.x) * kVec);
+}
+
+// Hoisted baby step (BSFusedKernel, src/extension/Hoist.cu:21-78): the ModUp
+// values of 4 coefficients are loaded once and reused by every rotation;
+// each rotation's products are accumulated and scattered to the
+// automorphism's positions d = brv17((brv17(x) + 1) g^-1 - 1). The grid is
+// XCD-aware: the 64 workgroups of one limb share an XCD (workgroup i runs on
+// XCD i mod 8), so the partial-line scatter writes of a limb merge in one
+// L2.
+constexpr int kBsBlocksPerLimb = 65536 / (kThreads * kVec);
+
+__device__ __forceinline__ void bs_fused_body(const EwNp &np, const PrimeTab &P,
+                                              const BsFusedArgs &A, int y,
+                                              int k) {
+  const int p = ew_prime(np, y);
+  const uint32_t q = P.q[p];
+  const int32_t qinv = P.qinv[p];
+  const size_t my = (size_t)y * 65536 + k;
+  const size_t kp = (size_t)p * 65536 + k;
+  const size_t tot_stride = (size_t)np.num_total * 65536;
+  // register arrays: every loop over groups is unrolled to the static bound
+  uint32_t mu[kMaxBsGroups][4];
+#pragma unroll
+  for (int gi = 0; gi < kMaxBsGroups; gi++) {
+    if (gi >= A.ngroups) break;
+    const bool own = (y >= A.own0[gi] && y < A.own1[gi]);
+    const uint4 v = own ? ld4(A.x + my)
+                        : ld4(A.modup + (size_t)A.group[gi] * tot_stride + my);
+    mu[gi][0] = v.x;
+    mu[gi][1] = v.y;
+    mu[gi][2] = v.z;
+    mu[gi][3] = v.w;
+  }
+  uint32_t pb[4] = {0, 0, 0, 0};
+  if (y < np.num_q) {
+    const uint32_t pp = A.p_prod[p];
+    const uint4 v = ld4(A.b + my);
+    pb[0] = mont_mul(pp, v.x, q, qinv);
+    pb[1] = mont_mul(pp, v.y, q, qinv);
+    pb[2] = mont_mul(pp, v.z, q, qinv);
+    pb[3] = mont_mul(pp, v.w, q, qinv);
+  }
+  uint32_t d[4];
+  for (int r = 0; r < A.nrot; r++) {
+    uint32_t rb[4] = {pb[0], pb[1], pb[2], pb[3]}, ra[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int gi = 0; gi < kMaxBsGroups; gi++) {
+      if (gi >= A.ngroups) break;
+      const uint4 kb = ld4(A.key_b[r][gi] + kp);
+      const uint4 ka = ld4(A.key_a[r][gi] + kp);
+      const uint32_t KB[4] = {kb.x, kb.y, kb.z, kb.w};
+      const uint32_t KA[4] = {ka.x, ka.y, ka.z, ka.w};
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        rb[i] = add_mod(rb[i], mont_mul(mu[gi][i], KB[i], q, qinv), q);
+        ra[i] = add_mod(ra[i], mont_mul(mu[gi][i], KA[i], q, qinv), q);
+      }
+    }
+    const uint32_t g = A.ginv[r];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const uint32_t e = (__brev((uint32_t)(k + i)) >> 15) + 1u;
+      d[i] = __brev((e * g - 1u) & 0x1FFFFu) >> 15;
+    }
+    uint32_t *ob = A.out_b[r] + (size_t)y * 65536;
+    uint32_t *oa = A.out_a[r] + (size_t)y * 65536;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      ob[d[i]] = rb[i];
+      oa[d[i]] = ra[i];
+    }
+  }
+}
+
+// limb slot of workgroup blockIdx.x for `nlimbs` limbs (XCD-aware order);
+// returns -1 for the padding workgroups of the last group of 8 limbs
+__device__ __forceinline__ int bs_limb(int nlimbs, int &chunk) {
+  const int xcd = blockIdx.x & 7;
+  const int j = blockIdx.x >> 3;
+  chunk = j % kBsBlocksPerLimb;
+  const int y = (j / kBsBlocksPerLimb) * 8 + xcd;
+  return y < nlimbs ? y : -1;
+}
+
+__global__ __launch_bounds__(kThreads) void bs_fused_kernel(
+    const EwNp np, const PrimeTab P, const BsFusedArgs A) {
+  int chunk;
+  const int y = bs_limb(np.num_total, chunk);
+  if (y < 0) return;
+  bs_fused_body(np, P, A, y, (chunk * kThreads + threadIdx.x) * kVec);
+}
+
+__global__ __launch_bounds__(kThreads) void bs_fused_table_kernel(
+    const BsFusedItem *__restrict__ items, const EwJob *__restrict__ jobs,
+    int njobs, const PrimeTab P) {
+  int chunk;
+  const int j = bs_limb(njobs, chunk);
+  if (j < 0) return;
+  const EwJob jb = jobs[j];
+  const BsFusedItem &it = items[jb.item];
+  bs_fused_body(it.np, P, it.A, jb.y, (chunk * kThreads + threadIdx.x) * kVec);
+}
+
 // Permute (ElementWise.cu:370-391): dst[i] = src[brv17((brv17(i)+1) g - 1)]
 // (log N + 1 = 17 bits; wraps mod 2N). Gathers are L2-served; writes are
 // coalesced.
@@ -272,29 +391,51 @@ void launch_const(const Context &ctx, int op, const EwNp &np,
 void launch_tensor(const Context &ctx, const EwNp &np, const EwOperand &c1,
                    const EwOperand &c2, const EwOperand &out, hipStream_t st) {
   if (np.num_total <= 0) return;
-  const int t = prof_begin(ctx, "tensor", np.num_total, st);
-  hipLaunchKernelGGL(tensor_kernel, ew_grid(np), dim3(kThreads), 0, st, np,
-                     ctx.ptab, c1, c2, out);
-  prof_end(ctx, t, st);
+  launch(ctx, "tensor", np.num_total, tensor_kernel, ew_grid(np),
+         dim3(kThreads), st, np, ctx.ptab, c1, c2, out);
 }
 
 void launch_key_accum(const Context &ctx, const EwNp &np,
                       const KeyAccumArgs &A, hipStream_t st) {
   if (np.num_total <= 0) return;
-  const int t = prof_begin(ctx, "key_accum", np.num_total, st);
-  hipLaunchKernelGGL(key_accum_kernel, ew_grid(np), dim3(kThreads), 0, st, np,
-                     ctx.ptab, A);
-  prof_end(ctx, t, st);
+  launch(ctx, "key_accum", np.num_total, key_accum_kernel, ew_grid(np),
+         dim3(kThreads), st, np, ctx.ptab, A);
 }
 
 void launch_permute(const Context &ctx, const EwNp &np, const EwOperand &x,
                     const EwOperand &out, uint32_t g, int npoly,
                     hipStream_t st) {
   if (np.num_total <= 0) return;
-  const int t = prof_begin(ctx, "permute", np.num_total, st);
-  hipLaunchKernelGGL(permute_kernel, dim3(65536 / kThreads, np.num_total),
-                     dim3(kThreads), 0, st, np, x, out, g, npoly);
-  prof_end(ctx, t, st);
+  launch(ctx, "permute", np.num_total, permute_kernel,
+         dim3(65536 / kThreads, np.num_total), dim3(kThreads), st, np, x, out,
+         g, npoly);
+}
+
+static int bs_grid(int nlimbs) {
+  return ((nlimbs + 7) / 8) * 8 * kBsBlocksPerLimb;
+}
+
+void launch_bs_fused(const Context &ctx, const EwNp &np, const BsFusedArgs &A,
+                     hipStream_t st) {
+  if (np.num_total <= 0 || A.nrot <= 0) return;
+  launch(ctx, "bs_fused", np.num_total, bs_fused_kernel,
+         dim3(bs_grid(np.num_total)), dim3(kThreads), st, np, ctx.ptab, A);
+}
+
+void launch_key_accum_table(const Context &ctx, const KeyAccumItem *items,
+                            const EwJob *jobs, int njobs, hipStream_t st) {
+  if (njobs <= 0) return;
+  launch(ctx, "key_accum[batch]", njobs, key_accum_table_kernel,
+         dim3(kBlocksPerLimb, njobs), dim3(kThreads), st, items, jobs,
+         ctx.ptab);
+}
+
+void launch_bs_fused_table(const Context &ctx, const BsFusedItem *items,
+                           const EwJob *jobs, int njobs, hipStream_t st) {
+  if (njobs <= 0) return;
+  launch(ctx, "bs_fused[batch]", njobs, bs_fused_table_kernel,
+         dim3(bs_grid(njobs)), dim3(kThreads), st, items, jobs, njobs,
+         ctx.ptab);
 }
 
 }  // namespace ck

@@ -4,6 +4,7 @@
 // (ck_*.hip) and the host orchestration (ck_context.cpp, ck_ops.cpp).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -45,6 +46,21 @@ enum NttFlags : int {
   kNttBconvInput = 32,      // forward: input = BaseConv of bsrc (see below)
 };
 
+// Per-limb fields of a kernel-argument plan, kept together in one record
+// (24 bytes, dword aligned): hipcc for gfx950 has folded a byte-indexed base
+// into the scalar load of a separate dword array (s_load with a base that
+// is not dword aligned, whose low bits the hardware drops), reading the
+// constant of the neighbouring limb.
+struct NttLimb {
+  uint32_t c1, c2, c3;  // plain-form constants (already carry R^-1)
+  uint16_t src_limb, dst_limb;
+  int16_t src2_limb;
+  uint16_t btab_off;
+  uint8_t prime, sel;
+  uint16_t pad_;
+};
+static_assert(sizeof(NttLimb) == 24, "NttLimb is one 24-byte record");
+
 struct NttLaunch {
   // up to four base pointers per role; limb i uses src[sel & 3],
   // dst[(sel >> 2) & 3], src2[(sel >> 4) & 3], bsrc[(sel >> 6) & 3]
@@ -52,8 +68,8 @@ struct NttLaunch {
   uint32_t *dst[4];
   const uint32_t *src2[4];
   // kNttBconvInput: the transform input of limb i is
-  //   sum_j bsrc[j] * T_j  mod q, T_j = B[btab_off[i] + 2j] * 2^16 +
-  //   B[btab_off[i] + 2j + 1], B = btab[(sel >> 6) & 3]
+  //   sum_j bsrc[j] * T_j  mod q, T_j = B[btab_off + 2j] * 2^16 +
+  //   B[btab_off + 2j + 1], B = btab[(sel >> 6) & 3]
   // over bs[(sel >> 6) & 3] contiguous centred int32 source limbs
   // (BaseConv, exact)
   const int32_t *bsrc[4];
@@ -61,16 +77,25 @@ struct NttLaunch {
   int bs[4];
   int n;
   int flags;
-  uint8_t prime[kMaxLaunchLimbs];
-  uint8_t sel[kMaxLaunchLimbs];
-  uint16_t src_limb[kMaxLaunchLimbs];
-  uint16_t dst_limb[kMaxLaunchLimbs];
-  int16_t src2_limb[kMaxLaunchLimbs];
-  uint16_t btab_off[kMaxLaunchLimbs];
-  uint32_t c1[kMaxLaunchLimbs];  // plain-form constants (already carry R^-1)
-  uint32_t c2[kMaxLaunchLimbs];
-  uint32_t c3[kMaxLaunchLimbs];
+  NttLimb limb[kMaxLaunchLimbs];
 };
+
+// One limb of a transform pass with every pointer resolved; the form a
+// batched plan keeps in device memory (one table per pass, 64 bytes a limb).
+// src2 == nullptr: no product operand / no kept limb; bsrc == nullptr: no
+// BaseConv input. flags as NttLaunch::flags.
+struct LimbJob {
+  const uint32_t *src;
+  uint32_t *dst;
+  const uint32_t *src2;
+  const int32_t *bsrc;
+  const double *btab;  // this output limb's row: bs x {hi, lo}
+  uint32_t c1, c2, c3;
+  uint16_t prime, bs;
+  int32_t flags;
+  int32_t pad_;
+};
+static_assert(sizeof(LimbJob) == 64, "LimbJob is one 64-byte record");
 
 constexpr int kMaxBconvSrc = 16;  // source limbs of one BaseConv
 constexpr int kMaxBconvDst = kMaxLaunchLimbs;
@@ -146,10 +171,22 @@ struct Context {
   mutable std::vector<std::string> prof_names;
 };
 
-// Per-kernel timing hooks: prof_begin returns a token (-1 when profiling is
-// off) that prof_end closes; both only record events on the stream.
-int prof_begin(const Context &c, const char *kernel, int limbs, hipStream_t st);
-void prof_end(const Context &c, int token, hipStream_t st);
+// Per-kernel timing: while profiling is on, prof_events hands out an event
+// pair for the next launch of `kernel` over `limbs` limbs, and launch()
+// passes it to hipExtLaunchKernelGGL, which stamps the dispatch's own start
+// and end into the events (no extra packets between kernels).
+bool prof_events(const Context &c, const char *kernel, int limbs,
+                 hipEvent_t *start, hipEvent_t *stop);
+
+template <typename F, typename... Args>
+inline void launch(const Context &c, const char *name, int limbs, F kernel,
+                   dim3 grid, dim3 block, hipStream_t st, Args... args) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (prof_events(c, name, limbs, &e0, &e1))
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, st, e0, e1, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+}
 int prof_report(const Context &c, std::vector<std::string> &names,
                 std::vector<double> &total_ms, std::vector<long> &calls);
 void prof_reset(const Context &c);
@@ -174,5 +211,11 @@ uint32_t h_to_mont(uint32_t a, uint32_t q);
 // ---- launchers ----
 void launch_ntt_fwd(const Context &ctx, const NttLaunch &L, hipStream_t st);
 void launch_intt(const Context &ctx, const NttLaunch &L, hipStream_t st);
+// table form: n jobs in device memory; flags_any = OR of the jobs' flags
+// (names the launch for profiling)
+void launch_ntt_fwd_table(const Context &ctx, const LimbJob *jobs, int n,
+                          int flags_any, hipStream_t st);
+void launch_intt_table(const Context &ctx, const LimbJob *jobs, int n,
+                       int flags_any, hipStream_t st);
 
 }  // namespace ck

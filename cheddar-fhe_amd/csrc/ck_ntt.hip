@@ -92,18 +92,36 @@ __device__ __forceinline__ int rlds(int rl, int b) {
   return rl * 256 + (b ^ ((b >> 4) & 15));
 }
 
-struct LimbView {
-  const uint32_t *src;
-  uint32_t *dst;
-  int p;
+// Launch descriptors: every kernel asks its descriptor for the LimbJob of
+// limb y (uniform per workgroup). JobByValue unpacks the kernel-argument
+// plan of a single operation; JobTable reads a batched plan's device table.
+struct JobByValue {
+  NttLaunch L;
+  __device__ __forceinline__ LimbJob job(int y) const {
+    const NttLimb &R = L.limb[y];
+    const int sel = R.sel;
+    const int b = (sel >> 6) & 3;
+    LimbJob J;
+    J.src = L.src[sel & 3] + (size_t)R.src_limb * kN;
+    J.dst = L.dst[(sel >> 2) & 3] + (size_t)R.dst_limb * kN;
+    const uint32_t *s2 = L.src2[(sel >> 4) & 3];
+    J.src2 = (s2 && R.src2_limb >= 0) ? s2 + (size_t)R.src2_limb * kN : nullptr;
+    J.bsrc = L.bsrc[b];
+    J.btab = L.btab[b] ? L.btab[b] + R.btab_off : nullptr;
+    J.bs = (uint16_t)L.bs[b];
+    J.c1 = R.c1;
+    J.c2 = R.c2;
+    J.c3 = R.c3;
+    J.prime = R.prime;
+    J.flags = L.flags;
+    return J;
+  }
 };
 
-__device__ __forceinline__ LimbView limb_view(const NttLaunch &L, int y) {
-  const int sel = L.sel[y];
-  return LimbView{L.src[sel & 3] + (size_t)L.src_limb[y] * kN,
-                  L.dst[(sel >> 2) & 3] + (size_t)L.dst_limb[y] * kN,
-                  L.prime[y]};
-}
+struct JobTable {
+  const LimbJob *__restrict__ jobs;
+  __device__ __forceinline__ LimbJob job(int y) const { return jobs[y]; }
+};
 
 // 16 KiB contiguous row tile <-> LDS, 16-byte accesses
 __device__ __forceinline__ void tile_rows_to_lds(const uint32_t *g, int32_t *lds) {
@@ -214,8 +232,9 @@ __device__ __forceinline__ int32_t lazy_i32(double x, const ModF &m) {
 // ---------------------------------------------------------------------------
 // Forward, column stages s = 0..7 -> lazy int32 intermediate
 // ---------------------------------------------------------------------------
+template <class D>
 __global__ __launch_bounds__(kColThreads) void ntt_fwd_cols(
-    const NttLaunch L, const PrimeTab P, const uint32_t *__restrict__ tw) {
+    const D desc, const PrimeTab P, const uint32_t *__restrict__ tw) {
   // int32 staging tile [256][32] aliased with the double exchange tile
   __shared__ double ldsd[256 * kColTile];
   int32_t *lds = reinterpret_cast<int32_t *>(ldsd);
@@ -224,10 +243,10 @@ __global__ __launch_bounds__(kColThreads) void ntt_fwd_cols(
   // tiles a BaseConv input shares stay in that XCD's L2.
   const int y = blockIdx.x >> 3;
   const int c0 = (blockIdx.x & 7) * kColTile;
-  const LimbView v = limb_view(L, y);
-  const double q = P.qd[v.p], qr = P.qr[v.p];
+  const LimbJob J = desc.job(y);
+  const double q = P.qd[J.prime], qr = P.qr[J.prime];
   const ModF m{q, qr};
-  const uint32_t *__restrict__ w = tw + (size_t)v.p * kN;
+  const uint32_t *__restrict__ w = tw + (size_t)J.prime * kN;
   const int cc = threadIdx.x & 31;
   const int rb = threadIdx.x >> 5;  // 0..15
 
@@ -241,18 +260,17 @@ __global__ __launch_bounds__(kColThreads) void ntt_fwd_cols(
       t1[(1 << s) - 1 + blk] = w[(1 << s) + blk];
       t2[(1 << s) - 1 + blk] = w[(16 << s) + (rb << s) + blk];
     }
-  if (L.flags & kNttBconvInput)
-    bconv_cols_to_lds(L.bsrc[(L.sel[y] >> 6) & 3], L.bs[(L.sel[y] >> 6) & 3],
-                      L.btab[(L.sel[y] >> 6) & 3] + L.btab_off[y], c0, m, lds);
+  if (J.flags & kNttBconvInput)
+    bconv_cols_to_lds(J.bsrc, J.bs, J.btab, c0, m, lds);
   else
-    tile_cols_to_lds(v.src, c0, lds);
+    tile_cols_to_lds(J.src, c0, lds);
   __syncthreads();
 
   double x[16];
 #pragma unroll
   for (int k = 0; k < 16; k++) x[k] = (double)lds[(rb + 16 * k) * 32 + cc];
-  if (L.flags & kNttPreMul) {
-    const double cv = (double)L.c1[y], cq = cv * qr;
+  if (J.flags & kNttPreMul) {
+    const double cv = (double)J.c1, cq = cv * qr;
 #pragma unroll
     for (int k = 0; k < 16; k++) x[k] = mulmod_f(x[k], cv, cq, q);
   }
@@ -268,21 +286,22 @@ __global__ __launch_bounds__(kColThreads) void ntt_fwd_cols(
 #pragma unroll
   for (int j = 0; j < 16; j++) lds[(rb * 16 + j) * 32 + cc] = lazy_i32(x[j], m);
   __syncthreads();
-  lds_to_tile_cols(lds, v.dst, c0);
+  lds_to_tile_cols(lds, J.dst, c0);
 }
 
 // ---------------------------------------------------------------------------
 // Forward, row stages s = 8..15 (+ optional ModDown epilogue)
 // ---------------------------------------------------------------------------
+template <class D>
 __global__ __launch_bounds__(kRowThreads) void ntt_fwd_rows(
-    const NttLaunch L, const PrimeTab P, const uint32_t *__restrict__ tw) {
+    const D desc, const PrimeTab P, const uint32_t *__restrict__ tw) {
   __shared__ double ldsd[kRowTile * 256];
   int32_t *lds = reinterpret_cast<int32_t *>(ldsd);
   const int y = blockIdx.y;
-  const LimbView v = limb_view(L, y);
-  const double q = P.qd[v.p], qr = P.qr[v.p];
+  const LimbJob J = desc.job(y);
+  const double q = P.qd[J.prime], qr = P.qr[J.prime];
   const ModF m{q, qr};
-  const uint32_t *__restrict__ w = tw + (size_t)v.p * kN;
+  const uint32_t *__restrict__ w = tw + (size_t)J.prime * kN;
   const int rl = threadIdx.x >> 4;
   const int t16 = threadIdx.x & 15;
   const int r = blockIdx.x * kRowTile + rl;  // row within the limb
@@ -299,8 +318,8 @@ __global__ __launch_bounds__(kRowThreads) void ntt_fwd_rows(
       t2[(1 << s) - 1 + blk] =
           w[(4096 << s) + (r << (4 + s)) + (t16 << s) + blk];
     }
-  // the pass-1 intermediate is lazy signed int32
-  tile_rows_to_lds(v.src + tile, lds);
+  // the pass-1 intermediate (in dst) is lazy signed int32
+  tile_rows_to_lds(J.dst + tile, lds);
   __syncthreads();
 
   double x[16];
@@ -317,14 +336,12 @@ __global__ __launch_bounds__(kRowThreads) void ntt_fwd_rows(
 
   // this thread now holds the 16 consecutive outputs b = 16 t16 + j of row r
   uint32_t out[16];
-  if (L.flags & kNttModDownEpilogue) {
+  if (J.flags & kNttModDownEpilogue) {
     // NTTPhase2ForModDown (NTT.cu:456-601): out = (src2 * pad - y) * ip
-    const int s2 = L.src2_limb[y];
-    const double ip = (double)L.c3[y], ipq = ip * qr;
+    const double ip = (double)J.c3, ipq = ip * qr;
     double a2[16];
-    if (s2 >= 0) {
-      const uint32_t *__restrict__ sp = L.src2[(L.sel[y] >> 4) & 3] +
-                                        (size_t)s2 * kN + r * 256 + t16 * 16;
+    if (J.src2) {
+      const uint32_t *__restrict__ sp = J.src2 + r * 256 + t16 * 16;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const uint4 u = reinterpret_cast<const uint4 *>(sp)[i];
@@ -333,8 +350,8 @@ __global__ __launch_bounds__(kRowThreads) void ntt_fwd_rows(
         a2[4 * i + 2] = (double)u.z;
         a2[4 * i + 3] = (double)u.w;
       }
-      if (L.flags & kNttEpiloguePad) {
-        const double pv = (double)L.c2[y], pq = pv * qr;
+      if (J.flags & kNttEpiloguePad) {
+        const double pv = (double)J.c2, pq = pv * qr;
 #pragma unroll
         for (int j = 0; j < 16; j++) a2[j] = mulmod_f(a2[j], pv, pq, q);
       }
@@ -349,7 +366,7 @@ __global__ __launch_bounds__(kRowThreads) void ntt_fwd_rows(
 #pragma unroll
     for (int j = 0; j < 16; j++) out[j] = canon_f(x[j], m);
   }
-  uint4 *o4 = reinterpret_cast<uint4 *>(v.dst + (size_t)r * 256 + t16 * 16);
+  uint4 *o4 = reinterpret_cast<uint4 *>(J.dst + (size_t)r * 256 + t16 * 16);
 #pragma unroll
   for (int i = 0; i < 4; i++)
     o4[i] = make_uint4(out[4 * i], out[4 * i + 1], out[4 * i + 2],
@@ -359,15 +376,16 @@ __global__ __launch_bounds__(kRowThreads) void ntt_fwd_rows(
 // ---------------------------------------------------------------------------
 // Inverse, row stages u = 0..7 -> lazy int32 intermediate
 // ---------------------------------------------------------------------------
+template <class D>
 __global__ __launch_bounds__(kRowThreads) void intt_rows(
-    const NttLaunch L, const PrimeTab P, const uint32_t *__restrict__ tw) {
+    const D desc, const PrimeTab P, const uint32_t *__restrict__ tw) {
   __shared__ double ldsd[kRowTile * 256];
   int32_t *lds = reinterpret_cast<int32_t *>(ldsd);
   const int y = blockIdx.y;
-  const LimbView v = limb_view(L, y);
-  const double q = P.qd[v.p], qr = P.qr[v.p];
+  const LimbJob J = desc.job(y);
+  const double q = P.qd[J.prime], qr = P.qr[J.prime];
   const ModF m{q, qr};
-  const uint32_t *__restrict__ w = tw + (size_t)v.p * kN;
+  const uint32_t *__restrict__ w = tw + (size_t)J.prime * kN;
   const int rl = threadIdx.x >> 4;
   const int t16 = threadIdx.x & 15;
   const int r = blockIdx.x * kRowTile + rl;
@@ -388,14 +406,13 @@ __global__ __launch_bounds__(kRowThreads) void intt_rows(
   double x[16];
   {
     const uint4 *s4 =
-        reinterpret_cast<const uint4 *>(v.src + (size_t)r * 256 + t16 * 16);
-    if (L.flags & kInttLoadProduct) {
+        reinterpret_cast<const uint4 *>(J.src + (size_t)r * 256 + t16 * 16);
+    if (J.flags & kInttLoadProduct) {
       // Tensor's a1 (*) a2 (ElementWise.cu:504-546) fused into the load
-      const uint32_t qu = P.q[v.p];
-      const int32_t qinv = P.qinv[v.p];
+      const uint32_t qu = P.q[J.prime];
+      const int32_t qinv = P.qinv[J.prime];
       const uint4 *t4 = reinterpret_cast<const uint4 *>(
-          L.src2[(L.sel[y] >> 4) & 3] + (size_t)L.src2_limb[y] * kN +
-          (size_t)r * 256 + t16 * 16);
+          J.src2 + (size_t)r * 256 + t16 * 16);
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const uint4 u = s4[i], w2 = t4[i];
@@ -427,22 +444,23 @@ __global__ __launch_bounds__(kRowThreads) void intt_rows(
 #pragma unroll
   for (int k = 0; k < 16; k++) lds[rlds(rl, t16 + 16 * k)] = lazy_i32(x[k], m);
   __syncthreads();
-  lds_to_tile_rows(lds, v.dst + tile);
+  lds_to_tile_rows(lds, J.dst + tile);
 }
 
 // ---------------------------------------------------------------------------
 // Inverse, column stages u = 8..15, then the per-limb constant
 // ---------------------------------------------------------------------------
+template <class D>
 __global__ __launch_bounds__(kColThreads) void intt_cols(
-    const NttLaunch L, const PrimeTab P, const uint32_t *__restrict__ tw) {
+    const D desc, const PrimeTab P, const uint32_t *__restrict__ tw) {
   __shared__ double ldsd[256 * kColTile];
   int32_t *lds = reinterpret_cast<int32_t *>(ldsd);
   const int y = blockIdx.x >> 3;  // same XCD-aware order as ntt_fwd_cols
   const int c0 = (blockIdx.x & 7) * kColTile;
-  const LimbView v = limb_view(L, y);
-  const double q = P.qd[v.p], qr = P.qr[v.p];
+  const LimbJob J = desc.job(y);
+  const double q = P.qd[J.prime], qr = P.qr[J.prime];
   const ModF m{q, qr};
-  const uint32_t *__restrict__ w = tw + (size_t)v.p * kN;
+  const uint32_t *__restrict__ w = tw + (size_t)J.prime * kN;
   const int cc = threadIdx.x & 31;
   const int rb = threadIdx.x >> 5;
 
@@ -456,8 +474,8 @@ __global__ __launch_bounds__(kColThreads) void intt_cols(
       t1[16 - (16 >> u) + blk] = w[(128 >> u) + (rb << (3 - u)) + blk];
       t2[16 - (16 >> u) + blk] = w[(8 >> u) + blk];
     }
-  // the pass-1 intermediate is lazy signed int32
-  tile_cols_to_lds(v.src, c0, lds);
+  // the pass-1 intermediate (in dst) is lazy signed int32
+  tile_cols_to_lds(J.dst, c0, lds);
   __syncthreads();
   double x[16];
 #pragma unroll
@@ -471,8 +489,8 @@ __global__ __launch_bounds__(kColThreads) void intt_cols(
 #pragma unroll
   for (int k = 0; k < 16; k++) x[k] = ldsd[(rb + 16 * k) * 32 + cc];
   gs16(x, t2, q, qr);
-  const double cv = (double)L.c1[y], cq = cv * qr;
-  const bool centred = (L.flags & kInttCentred) != 0;
+  const double cv = (double)J.c1, cq = cv * qr;
+  const bool centred = (J.flags & kInttCentred) != 0;
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < 16; k++) {
@@ -484,16 +502,35 @@ __global__ __launch_bounds__(kColThreads) void intt_cols(
     lds[(rb + 16 * k) * 32 + cc] = (int32_t)r1;
   }
   __syncthreads();
-  lds_to_tile_cols(lds, v.dst, c0);
+  lds_to_tile_cols(lds, J.dst, c0);
 }
 
-// second pass reads what the first pass wrote
-void in_place(NttLaunch &L) {
-  for (int i = 0; i < 4; i++) L.src[i] = L.dst[i];
-  for (int i = 0; i < L.n; i++) {
-    L.src_limb[i] = L.dst_limb[i];
-    L.sel[i] = (uint8_t)((L.sel[i] & ~3) | ((L.sel[i] >> 2) & 3));
-  }
+const char *fwd_cols_name(int flags) {
+  return (flags & kNttBconvInput) ? "ntt_fwd_cols+bconv" : "ntt_fwd_cols";
+}
+const char *fwd_rows_name(int flags) {
+  return (flags & kNttModDownEpilogue) ? "ntt_fwd_rows+epilogue" : "ntt_fwd_rows";
+}
+const char *inv_rows_name(int flags) {
+  return (flags & kInttLoadProduct) ? "intt_rows+tensor" : "intt_rows";
+}
+
+template <class D>
+void fwd_passes(const Context &ctx, const D &d, int n, int flags,
+                hipStream_t st) {
+  launch(ctx, fwd_cols_name(flags), n, ntt_fwd_cols<D>, dim3(kColTiles * n),
+         dim3(kColThreads), st, d, ctx.ptab, (const uint32_t *)ctx.d_tw_fwd);
+  launch(ctx, fwd_rows_name(flags), n, ntt_fwd_rows<D>, dim3(kRowTiles, n),
+         dim3(kRowThreads), st, d, ctx.ptab, (const uint32_t *)ctx.d_tw_fwd);
+}
+
+template <class D>
+void inv_passes(const Context &ctx, const D &d, int n, int flags,
+                hipStream_t st) {
+  launch(ctx, inv_rows_name(flags), n, intt_rows<D>, dim3(kRowTiles, n),
+         dim3(kRowThreads), st, d, ctx.ptab, (const uint32_t *)ctx.d_tw_inv);
+  launch(ctx, "intt_cols", n, intt_cols<D>, dim3(kColTiles * n),
+         dim3(kColThreads), st, d, ctx.ptab, (const uint32_t *)ctx.d_tw_inv);
 }
 
 }  // namespace
@@ -502,35 +539,26 @@ void in_place(NttLaunch &L) {
 // on dst (with the optional ModDown epilogue).
 void launch_ntt_fwd(const Context &ctx, const NttLaunch &L, hipStream_t st) {
   if (L.n <= 0) return;
-  int t = prof_begin(ctx, (L.flags & kNttBconvInput) ? "ntt_fwd_cols+bconv"
-                                                     : "ntt_fwd_cols", L.n, st);
-  hipLaunchKernelGGL(ntt_fwd_cols, dim3(kColTiles * L.n), dim3(kColThreads), 0,
-                     st, L, ctx.ptab, ctx.d_tw_fwd);
-  prof_end(ctx, t, st);
-  NttLaunch L2 = L;
-  in_place(L2);
-  t = prof_begin(ctx, (L.flags & kNttModDownEpilogue) ? "ntt_fwd_rows+epilogue"
-                                                      : "ntt_fwd_rows", L.n, st);
-  hipLaunchKernelGGL(ntt_fwd_rows, dim3(kRowTiles, L.n), dim3(kRowThreads), 0,
-                     st, L2, ctx.ptab, ctx.d_tw_fwd);
-  prof_end(ctx, t, st);
+  fwd_passes(ctx, JobByValue{L}, L.n, L.flags, st);
 }
 
 // Inverse NTT of L.n limbs: row stages into dst, then column stages in
 // place on dst with the post constant c1 (plain form) and optional centring.
 void launch_intt(const Context &ctx, const NttLaunch &L, hipStream_t st) {
   if (L.n <= 0) return;
-  int t = prof_begin(ctx, (L.flags & kInttLoadProduct) ? "intt_rows+tensor"
-                                                       : "intt_rows", L.n, st);
-  hipLaunchKernelGGL(intt_rows, dim3(kRowTiles, L.n), dim3(kRowThreads), 0, st,
-                     L, ctx.ptab, ctx.d_tw_inv);
-  prof_end(ctx, t, st);
-  NttLaunch L2 = L;
-  in_place(L2);
-  t = prof_begin(ctx, "intt_cols", L.n, st);
-  hipLaunchKernelGGL(intt_cols, dim3(kColTiles * L.n), dim3(kColThreads), 0, st,
-                     L2, ctx.ptab, ctx.d_tw_inv);
-  prof_end(ctx, t, st);
+  inv_passes(ctx, JobByValue{L}, L.n, L.flags, st);
+}
+
+void launch_ntt_fwd_table(const Context &ctx, const LimbJob *jobs, int n,
+                          int flags_any, hipStream_t st) {
+  if (n <= 0) return;
+  fwd_passes(ctx, JobTable{jobs}, n, flags_any, st);
+}
+
+void launch_intt_table(const Context &ctx, const LimbJob *jobs, int n,
+                       int flags_any, hipStream_t st) {
+  if (n <= 0) return;
+  inv_passes(ctx, JobTable{jobs}, n, flags_any, st);
 }
 
 }  // namespace ck

@@ -1,12 +1,12 @@
-// Host orchestration of the key-switching path on one HIP stream:
-// ModUp, ModDown / Rescale / ModDownAndRescale, the key inner product and
-// the evaluator compositions HMult, Relinearize, MultKey, HRot, HConj,
-// Rescale (reference: src/core/ModSwitch.cu:413-684,
-// src/core/Context.cpp:316-871).
+// Host orchestration of the key-switching path: ModUp, ModDown / Rescale /
+// ModDownAndRescale, the key inner product and the evaluator compositions
+// HMult, Relinearize, MultKey, HRot, HConj and hoisted rotations
+// (reference: src/core/ModSwitch.cu:413-684, src/core/Context.cpp:316-871,
+// src/extension/Hoist.cu:207-327, :578-691).
 //
-// Every launch plan is built on the host and passed by value; temporaries
-// come from the context workspace, so an operation issues no allocation, no
-// host<->device copy and no synchronisation.
+// Every launch plan is built on the host; temporaries come from a caller-
+// provided workspace, so an operation issues no allocation, no host<->device
+// copy and no synchronisation.
 //
 // Fusions relative to the reference's kernel sequence (results identical):
 //   * BaseConv runs inside the forward NTT's column pass (ModUp and ModDown),
@@ -14,7 +14,8 @@
 //     and d0 = b1 (*) b2, d1 = b1 (*) a2 + a1 (*) b2 inside the key inner
 //     product's P * (d0, d1) fold,
 //   * ModUp's own-group limbs are read from the source by the inner product
-//     instead of being copied.
+//     instead of being copied,
+//   * all polynomials of a ModDown share its launches.
 #include "ck_ops.h"
 
 #include <algorithm>
@@ -30,92 +31,152 @@ int launch_status(const char *what) {
   return check_hip(hipGetLastError(), what);
 }
 
-struct NttJob {
-  int prime, sel, src_limb, dst_limb, src2_limb;
-  uint32_t c1, c2, c3;
-  int btab_off;
-};
-
-// Split a list of limb jobs into launches of at most kMaxLaunchLimbs.
-void run_jobs(const Context &c, bool forward, const NttLaunch &base,
-              const std::vector<NttJob> &jobs, hipStream_t st) {
-  for (size_t off = 0; off < jobs.size(); off += kMaxLaunchLimbs) {
-    NttLaunch L = base;
-    L.n = (int)std::min<size_t>(kMaxLaunchLimbs, jobs.size() - off);
-    for (int i = 0; i < L.n; i++) {
-      const NttJob &j = jobs[off + i];
-      L.prime[i] = (uint8_t)j.prime;
-      L.sel[i] = (uint8_t)j.sel;
-      L.src_limb[i] = (uint16_t)j.src_limb;
-      L.dst_limb[i] = (uint16_t)j.dst_limb;
-      L.src2_limb[i] = (int16_t)j.src2_limb;
-      L.btab_off[i] = (uint16_t)j.btab_off;
-      L.c1[i] = j.c1;
-      L.c2[i] = j.c2;
-      L.c3[i] = j.c3;
-    }
-    if (forward)
-      launch_ntt_fwd(c, L, st);
-    else
-      launch_intt(c, L, st);
-  }
-}
-
-NttLaunch empty_launch() {
-  NttLaunch L;
-  std::memset(&L, 0, sizeof(L));
-  return L;
-}
-
 uint32_t r_inv_mod(uint32_t q) { return h_invmod(h_to_mont(1, q), q); }
 
+// slot of `ptr` among the first `n` entries of `arr` (appended if absent and
+// there is room); -1 when all four slots hold other pointers
+template <typename T>
+int slot_of(T *arr, int &n, T ptr) {
+  for (int i = 0; i < n; i++)
+    if (arr[i] == ptr) return i;
+  if (n == 4) return -1;
+  arr[n] = ptr;
+  return n++;
+}
+
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// ImmediateSink
+// ---------------------------------------------------------------------------
+
+int ImmediateSink::pass(bool forward, int flags,
+                        const std::vector<PassJob> &jobs) {
+  NttLaunch L;
+  int ns = 0, nd = 0, n2 = 0, nb = 0;
+  auto reset = [&]() {
+    std::memset(&L, 0, sizeof(L));
+    L.flags = flags;
+    ns = nd = n2 = nb = 0;
+  };
+  auto flush = [&]() {
+    if (L.n == 0) return;
+    if (forward)
+      launch_ntt_fwd(c_, L, st_);
+    else
+      launch_intt(c_, L, st_);
+    reset();
+  };
+  reset();
+  for (const PassJob &j : jobs) {
+    for (int attempt = 0;; attempt++) {
+      int a = slot_of(L.src, ns, j.src);
+      int b = slot_of(L.dst, nd, j.dst);
+      int s2 = slot_of(L.src2, n2, j.src2);
+      // BaseConv sources and table go together
+      int bc = -1;
+      for (int i = 0; i < nb; i++)
+        if (L.bsrc[i] == j.bsrc && L.btab[i] == j.btab && L.bs[i] == j.bs)
+          bc = i;
+      if (bc < 0 && nb < 4) {
+        L.bsrc[nb] = j.bsrc;
+        L.btab[nb] = j.btab;
+        L.bs[nb] = j.bs;
+        bc = nb++;
+      }
+      if (a >= 0 && b >= 0 && s2 >= 0 && bc >= 0 && L.n < kMaxLaunchLimbs) {
+        NttLimb &R = L.limb[L.n++];
+        R.prime = (uint8_t)j.prime;
+        R.sel = (uint8_t)(a | (b << 2) | (s2 << 4) | (bc << 6));
+        R.src_limb = (uint16_t)j.src_limb;
+        R.dst_limb = (uint16_t)j.dst_limb;
+        R.src2_limb = (int16_t)(j.src2 ? j.src2_limb : -1);
+        R.btab_off = (uint16_t)j.btab_off;
+        R.c1 = j.c1;
+        R.c2 = j.c2;
+        R.c3 = j.c3;
+        break;
+      }
+      if (attempt > 0) {
+        set_error("transform job does not fit an empty launch");
+        return kErrUnsupported;
+      }
+      flush();
+    }
+  }
+  flush();
+  return launch_status(forward ? "ntt" : "intt");
+}
+
+int ImmediateSink::copy(uint32_t *dst, const uint32_t *src, size_t words) {
+  return check_hip(hipMemcpyAsync(dst, src, words * sizeof(uint32_t),
+                                  hipMemcpyDeviceToDevice, st_),
+                   "copy");
+}
+
+int ImmediateSink::key_accum(const EwNp &np, const KeyAccumArgs &A) {
+  launch_key_accum(c_, np, A, st_);
+  return launch_status("key accumulate");
+}
+
+int ImmediateSink::bs_fused(const EwNp &np, const BsFusedArgs &A) {
+  launch_bs_fused(c_, np, A, st_);
+  return launch_status("hoisted key accumulate");
+}
+
+int ImmediateSink::permute(const EwNp &np, const EwOperand &x,
+                           const EwOperand &out, uint32_t g, int npoly) {
+  launch_permute(c_, np, x, out, g, npoly, st_);
+  return launch_status("permute");
+}
 
 // ---------------------------------------------------------------------------
 // NTTHandler
 // ---------------------------------------------------------------------------
 
 int op_ntt(const Context &c, uint32_t *dst, const uint32_t *src, int nm,
-           int nt, int na, int src_gap, bool mont_conv, hipStream_t st) {
-  NttLaunch L = empty_launch();
-  L.src[0] = src;
-  L.dst[0] = dst;
-  L.flags = mont_conv ? kNttPreMul : 0;
+           int nt, int na, int src_gap, bool mont_conv, Sink &s) {
   const int num_q = nm + nt;
-  std::vector<NttJob> jobs;
+  std::vector<PassJob> jobs;
   for (int t = 0; t < num_q + na; t++) {
     const int p = global_prime(c, nm, nt, t);
+    PassJob j;
+    j.src = src;
+    j.src_limb = t < num_q ? t : t + src_gap;
+    j.dst = dst;
+    j.dst_limb = t;
+    j.prime = p;
     // NTT(..., montgomery_conversion): x * R^2 * R^-1 = x * R
-    const uint32_t cv = mont_conv ? h_to_mont(1, c.primes[p]) : 1u;
-    jobs.push_back({p, 0, t < num_q ? t : t + src_gap, t, -1, cv, 0, 0, 0});
+    j.c1 = mont_conv ? h_to_mont(1, c.primes[p]) : 1u;
+    jobs.push_back(j);
   }
-  run_jobs(c, true, L, jobs, st);
-  return launch_status("ntt");
+  return s.pass(true, mont_conv ? kNttPreMul : 0, jobs);
 }
 
 // INTT followed by x * const * R^-1 with per-limb constants given in the
 // reference's convention (the value it passes to MultConst).
 int op_intt_const(const Context &c, uint32_t *dst, const uint32_t *src,
                   int nm, int nt, int na, int src_gap,
-                  const uint32_t *consts_mont, bool centred, hipStream_t st) {
-  NttLaunch L = empty_launch();
-  L.src[0] = src;
-  L.dst[0] = dst;
-  L.flags = centred ? kInttCentred : 0;
+                  const uint32_t *consts_mont, bool centred, Sink &s) {
   const int num_q = nm + nt;
-  std::vector<NttJob> jobs;
+  std::vector<PassJob> jobs;
   for (int t = 0; t < num_q + na; t++) {
     const int p = global_prime(c, nm, nt, t);
     const uint32_t q = c.primes[p];
-    const uint32_t cv = h_mulmod(consts_mont[t] % q, r_inv_mod(q), q);
-    jobs.push_back({p, 0, t < num_q ? t : t + src_gap, t, -1, cv, 0, 0, 0});
+    PassJob j;
+    j.src = src;
+    j.src_limb = t < num_q ? t : t + src_gap;
+    j.dst = dst;
+    j.dst_limb = t;
+    j.prime = p;
+    j.c1 = h_mulmod(consts_mont[t] % q, r_inv_mod(q), q);
+    jobs.push_back(j);
   }
-  run_jobs(c, false, L, jobs, st);
-  return launch_status("intt");
+  return s.pass(false, centred ? kInttCentred : 0, jobs);
 }
 
 int op_intt(const Context &c, uint32_t *dst, const uint32_t *src, int nm,
-            int nt, int na, int src_gap, bool mont_conv, hipStream_t st) {
+            int nt, int na, int src_gap, bool mont_conv, Sink &s) {
   // NTT.cu:673-741: N^-1 (leave Montgomery form) or N^-1 * R (keep it)
   std::vector<uint32_t> cs;
   for (int t = 0; t < nm + nt + na; t++) {
@@ -123,7 +184,7 @@ int op_intt(const Context &c, uint32_t *dst, const uint32_t *src, int nm,
     const uint32_t ninv = h_invmod((uint32_t)c.n % q, q);
     cs.push_back(mont_conv ? ninv : h_to_mont(ninv, q));
   }
-  return op_intt_const(c, dst, src, nm, nt, na, src_gap, cs.data(), false, st);
+  return op_intt_const(c, dst, src, nm, nt, na, src_gap, cs.data(), false, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -135,62 +196,59 @@ int op_intt(const Context &c, uint32_t *dst, const uint32_t *src, int nm,
 // src (*) prod2 (HMult's a1 (*) a2). copy_own = false leaves each group's own
 // limbs unwritten (the fused inner product reads them from the source).
 int op_modup(const Context &c, int level, uint32_t *dst, const uint32_t *src,
-             uint32_t *v, hipStream_t st, bool copy_own,
-             const uint32_t *prod2) {
+             uint32_t *v, Sink &s, bool copy_own, const uint32_t *prod2) {
   const LevelPlan &lp = c.levels[level];
   const int tot = lp.num_q + c.alpha;
   // 1. INTT of all q limbs with the per-group constant, centred output
   {
-    NttLaunch L = empty_launch();
-    L.src[0] = src;
-    L.src2[0] = prod2;
-    L.dst[0] = v;
-    L.flags = kInttCentred | (prod2 ? kInttLoadProduct : 0);
-    std::vector<NttJob> jobs;
-    for (int t = 0; t < lp.num_q; t++)
-      jobs.push_back({lp.q_prime[t], 0, t, t, t, lp.modup_c1[t], 0, 0, 0});
-    run_jobs(c, false, L, jobs, st);
+    std::vector<PassJob> jobs;
+    for (int t = 0; t < lp.num_q; t++) {
+      PassJob j;
+      j.src = src;
+      j.src_limb = t;
+      j.dst = v;
+      j.dst_limb = t;
+      j.src2 = prod2;
+      j.src2_limb = t;
+      j.prime = lp.q_prime[t];
+      j.c1 = lp.modup_c1[t];
+      jobs.push_back(j);
+    }
+    int rc = s.pass(false, kInttCentred | (prod2 ? kInttLoadProduct : 0), jobs);
+    if (rc) return rc;
   }
-  // 2. BaseConv of each group fused into the forward NTT of its outputs;
-  // up to four groups share a launch (one selector each)
-  NttLaunch L = empty_launch();
-  L.flags = kNttBconvInput;
-  std::vector<NttJob> jobs;
-  int nsel = 0;
-  auto flush = [&]() {
-    run_jobs(c, true, L, jobs, st);
-    jobs.clear();
-    nsel = 0;
-  };
+  // 2. BaseConv of each group fused into the forward NTT of its outputs
+  std::vector<PassJob> jobs;
   for (int g = 0; g < lp.beta; g++) {
     const ModUpGroup &gr = lp.groups[g];
     if (gr.s1 <= gr.s0) continue;
-    const int s = gr.s1 - gr.s0;
-    const int d = (int)gr.dst_limb.size();
-    if (nsel == 4 || jobs.size() + d > (size_t)kMaxLaunchLimbs) flush();
-    const int sel = nsel++;
-    L.src[sel] = dst;
-    L.dst[sel] = dst;
-    L.bsrc[sel] = reinterpret_cast<const int32_t *>(v + (size_t)gr.s0 * kLimb);
-    L.btab[sel] = gr.d_ftab;
-    L.bs[sel] = s;
-    for (int m = 0; m < d; m++) {
-      const int limb = g * tot + gr.dst_limb[m];
-      jobs.push_back({gr.dst_prime[m], sel | (sel << 2) | (sel << 6), limb, limb,
-                      -1, 1, 0, 0, 2 * m * s});
+    const int sl = gr.s1 - gr.s0;
+    for (size_t m = 0; m < gr.dst_limb.size(); m++) {
+      PassJob j;
+      j.src = dst;
+      j.dst = dst;
+      j.src_limb = j.dst_limb = g * tot + gr.dst_limb[m];
+      j.bsrc = reinterpret_cast<const int32_t *>(v + (size_t)gr.s0 * kLimb);
+      j.btab = gr.d_ftab;
+      j.bs = sl;
+      j.btab_off = 2 * (int)m * sl;
+      j.prime = gr.dst_prime[m];
+      j.c1 = 1;
+      jobs.push_back(j);
     }
-    if (copy_own) {
-      int rc = check_hip(
-          hipMemcpyAsync(dst + ((size_t)g * tot + gr.s0) * kLimb,
-                         src + (size_t)gr.s0 * kLimb,
-                         (size_t)s * kLimb * sizeof(uint32_t),
-                         hipMemcpyDeviceToDevice, st),
-          "modup copy");
+  }
+  int rc = s.pass(true, kNttBconvInput, jobs);
+  if (rc) return rc;
+  if (copy_own) {
+    for (int g = 0; g < lp.beta; g++) {
+      const ModUpGroup &gr = lp.groups[g];
+      if (gr.s1 <= gr.s0) continue;
+      rc = s.copy(dst + ((size_t)g * tot + gr.s0) * kLimb,
+                  src + (size_t)gr.s0 * kLimb, (size_t)(gr.s1 - gr.s0) * kLimb);
       if (rc) return rc;
     }
   }
-  if (!jobs.empty()) flush();
-  return launch_status("modup");
+  return kOk;
 }
 
 const ModDownPlan *moddown_plan(const Context &c, int level, int kind) {
@@ -202,61 +260,85 @@ const ModDownPlan *moddown_plan(const Context &c, int level, int kind) {
   return mp->valid ? mp : nullptr;
 }
 
-// ModDownWorker (ModSwitch.cu:566-684) for npoly (<= 4) polynomials at
-// once; v is a workspace of npoly * src_len limbs.
+// ModDownWorker (ModSwitch.cu:566-684) for npoly polynomials at once; v is
+// a workspace of npoly * src_len limbs.
 int op_moddown(const Context &c, int level, int kind, int npoly,
                uint32_t *const *dst, const uint32_t *const *src, uint32_t *v,
-               hipStream_t st) {
+               Sink &s) {
   const ModDownPlan *mp = moddown_plan(c, level, kind);
   if (!mp) {
     set_error("no ModDown plan for this level and kind");
     return kErrInvalidArg;
   }
-  if ((size_t)npoly * mp->src_len * kLimb > c.ws_words) {
-    set_error("ModDown workspace too small");
-    return kErrUnsupported;
-  }
   // 1. INTT of the removed limbs, x const1, centred
   {
-    NttLaunch L = empty_launch();
-    std::vector<NttJob> jobs;
-    for (int j = 0; j < npoly; j++) {
-      L.src[j] = src[j];
-      L.dst[j] = v + (size_t)j * mp->src_len * kLimb;
-      for (int i = 0; i < mp->src_len; i++)
-        jobs.push_back({mp->src_prime[i], j | (j << 2), mp->src_limb[i], i, -1,
-                        mp->c1[i], 0, 0, 0});
-    }
-    L.flags = kInttCentred;
-    run_jobs(c, false, L, jobs, st);
+    std::vector<PassJob> jobs;
+    for (int k = 0; k < npoly; k++)
+      for (int i = 0; i < mp->src_len; i++) {
+        PassJob j;
+        j.src = src[k];
+        j.src_limb = mp->src_limb[i];
+        j.dst = v + (size_t)k * mp->src_len * kLimb;
+        j.dst_limb = i;
+        j.prime = mp->src_prime[i];
+        j.c1 = mp->c1[i];
+        jobs.push_back(j);
+      }
+    int rc = s.pass(false, kInttCentred, jobs);
+    if (rc) return rc;
   }
   // 2. BaseConv fused into the forward NTT, 3. epilogue in the row pass
-  {
-    NttLaunch L = empty_launch();
-    L.flags = kNttBconvInput | kNttModDownEpilogue |
-              (mp->use_pad ? kNttEpiloguePad : 0);
-    std::vector<NttJob> jobs;
-    for (int j = 0; j < npoly; j++) {
-      L.src[j] = dst[j];
-      L.dst[j] = dst[j];
-      L.src2[j] = src[j];
-      L.bsrc[j] =
-          reinterpret_cast<const int32_t *>(v + (size_t)j * mp->src_len * kLimb);
-      L.btab[j] = mp->d_ftab;
-      L.bs[j] = mp->src_len;
-      for (int y = 0; y < mp->dst_len; y++)
-        jobs.push_back({mp->dst_prime[y], j | (j << 2) | (j << 4) | (j << 6), y,
-                        y, mp->src2_limb[y], 1, mp->pad[y], mp->inv_prod[y],
-                        2 * y * mp->src_len});
+  std::vector<PassJob> jobs;
+  for (int k = 0; k < npoly; k++)
+    for (int y = 0; y < mp->dst_len; y++) {
+      PassJob j;
+      j.src = dst[k];
+      j.dst = dst[k];
+      j.src_limb = j.dst_limb = y;
+      j.src2 = src[k];
+      j.src2_limb = mp->src2_limb[y];
+      j.bsrc = reinterpret_cast<const int32_t *>(v + (size_t)k * mp->src_len * kLimb);
+      j.btab = mp->d_ftab;
+      j.bs = mp->src_len;
+      j.btab_off = 2 * y * mp->src_len;
+      j.prime = mp->dst_prime[y];
+      j.c1 = 1;
+      j.c2 = mp->pad[y];
+      j.c3 = mp->inv_prod[y];
+      jobs.push_back(j);
     }
-    run_jobs(c, true, L, jobs, st);
-  }
-  return launch_status("moddown");
+  return s.pass(true,
+                kNttBconvInput | kNttModDownEpilogue |
+                    (mp->use_pad ? kNttEpiloguePad : 0),
+                jobs);
 }
 
 // ---------------------------------------------------------------------------
 // Context: key switching
 // ---------------------------------------------------------------------------
+
+namespace {
+
+// the ModUp groups that carry limbs at this level, in KeyAccumArgs form
+void fill_groups(const LevelPlan &lp, const uint32_t *const *key_b,
+                 const uint32_t *const *key_a, KeyAccumArgs &A) {
+  A.ngroups = 0;
+  for (int g = 0; g < lp.beta; g++) {
+    if (lp.groups[g].s1 <= lp.groups[g].s0) continue;
+    A.group[A.ngroups++] = g;
+    A.own0[g] = lp.groups[g].s0;
+    A.own1[g] = lp.groups[g].s1;
+    A.key_b[g] = key_b[g];
+    A.key_a[g] = key_a[g];
+  }
+}
+
+}  // namespace
+
+size_t mult_key_ws_limbs(const Context &c, int level) {
+  const LevelPlan &lp = c.levels[level];
+  return (size_t)lp.num_q + (size_t)lp.beta * (lp.num_q + c.alpha);
+}
 
 // MultKeyNoModDown (Context.cpp:609-712): ModUp(x), key inner product,
 // accum[q] += P * add. Workspace: v (num_q) + modup (beta * tot). With
@@ -267,25 +349,17 @@ int op_mult_key_no_moddown(const Context &c, int level, uint32_t *acc_b,
                            const uint32_t *add_b, const uint32_t *add_a,
                            const uint32_t *const *key_b,
                            const uint32_t *const *key_a, uint32_t *ws,
-                           hipStream_t st, const uint32_t *const *tensor) {
+                           Sink &s, const uint32_t *const *tensor) {
   const LevelPlan &lp = c.levels[level];
   uint32_t *v = ws;
   uint32_t *modup = ws + (size_t)lp.num_q * kLimb;
-  int rc = op_modup(c, level, modup, tensor ? tensor[1] : x, v, st, false,
+  int rc = op_modup(c, level, modup, tensor ? tensor[1] : x, v, s, false,
                     tensor ? tensor[3] : nullptr);
   if (rc) return rc;
   KeyAccumArgs A;
   std::memset(&A, 0, sizeof(A));
   A.modup = modup;
-  A.ngroups = 0;
-  for (int g = 0; g < lp.beta; g++) {
-    if (lp.groups[g].s1 <= lp.groups[g].s0) continue;
-    A.group[A.ngroups++] = g;
-    A.own0[g] = lp.groups[g].s0;
-    A.own1[g] = lp.groups[g].s1;
-    A.key_b[g] = key_b[g];
-    A.key_a[g] = key_a[g];
-  }
+  fill_groups(lp, key_b, key_a, A);
   A.x = x;
   A.add_b = add_b;
   A.add_a = add_a;
@@ -296,14 +370,14 @@ int op_mult_key_no_moddown(const Context &c, int level, uint32_t *acc_b,
   A.p_prod = c.d_p_prod;
   A.out_b = acc_b;
   A.out_a = acc_a;
-  EwNp np = make_np(c, lp.nm, lp.nt, c.alpha);
-  launch_key_accum(c, np, A, st);
-  return launch_status("key accumulate");
+  return s.key_accum(make_np(c, lp.nm, lp.nt, c.alpha), A);
 }
 
-size_t mult_key_ws_limbs(const Context &c, int level) {
+size_t relin_ws_limbs(const Context &c, int level) {
   const LevelPlan &lp = c.levels[level];
-  return (size_t)lp.num_q + (size_t)lp.beta * (lp.num_q + c.alpha);
+  const size_t tot = (size_t)lp.num_q + c.alpha;
+  // accumulators + max(ModUp workspace, ModDown workspace of two polys)
+  return 2 * tot + std::max(mult_key_ws_limbs(c, level), 2 * (size_t)tot);
 }
 
 // Relinearize / RelinearizeRescale of (d0, d1, d2) (Context.cpp:561-565,
@@ -312,18 +386,18 @@ int op_relinearize(const Context &c, int level, uint32_t *out_b,
                    uint32_t *out_a, const uint32_t *d0, const uint32_t *d1,
                    const uint32_t *d2, const uint32_t *const *key_b,
                    const uint32_t *const *key_a, bool rescale, uint32_t *ws,
-                   hipStream_t st) {
+                   Sink &s) {
   const LevelPlan &lp = c.levels[level];
   const int tot = lp.num_q + c.alpha;
   uint32_t *acc_b = ws;
   uint32_t *acc_a = ws + (size_t)tot * kLimb;
   uint32_t *rest = ws + 2 * (size_t)tot * kLimb;
   int rc = op_mult_key_no_moddown(c, level, acc_b, acc_a, d2, d0, d1, key_b,
-                                  key_a, rest, st);
+                                  key_a, rest, s);
   if (rc) return rc;
   uint32_t *dst[2] = {out_b, out_a};
   const uint32_t *src[2] = {acc_b, acc_a};
-  return op_moddown(c, level, rescale ? 2 : 0, 2, dst, src, rest, st);
+  return op_moddown(c, level, rescale ? 2 : 0, 2, dst, src, rest, s);
 }
 
 // HMult (Context.cpp:862-871): Tensor then Relinearize(Rescale), with the
@@ -331,19 +405,20 @@ int op_relinearize(const Context &c, int level, uint32_t *out_b,
 int op_hmult(const Context &c, int level, uint32_t *out_b, uint32_t *out_a,
              const uint32_t *b1, const uint32_t *a1, const uint32_t *b2,
              const uint32_t *a2, const uint32_t *const *key_b,
-             const uint32_t *const *key_a, bool rescale, hipStream_t st) {
+             const uint32_t *const *key_a, bool rescale, uint32_t *ws,
+             Sink &s) {
   const LevelPlan &lp = c.levels[level];
   const int tot = lp.num_q + c.alpha;
-  uint32_t *acc_b = c.ws;
-  uint32_t *acc_a = c.ws + (size_t)tot * kLimb;
-  uint32_t *rest = c.ws + 2 * (size_t)tot * kLimb;
+  uint32_t *acc_b = ws;
+  uint32_t *acc_a = ws + (size_t)tot * kLimb;
+  uint32_t *rest = ws + 2 * (size_t)tot * kLimb;
   const uint32_t *tensor[4] = {b1, a1, b2, a2};
   int rc = op_mult_key_no_moddown(c, level, acc_b, acc_a, nullptr, nullptr,
-                                  nullptr, key_b, key_a, rest, st, tensor);
+                                  nullptr, key_b, key_a, rest, s, tensor);
   if (rc) return rc;
   uint32_t *dst[2] = {out_b, out_a};
   const uint32_t *src[2] = {acc_b, acc_a};
-  return op_moddown(c, level, rescale ? 2 : 0, 2, dst, src, rest, st);
+  return op_moddown(c, level, rescale ? 2 : 0, 2, dst, src, rest, s);
 }
 
 // MultKey on (b, a) (Context.cpp:584-607): key-switch a, accum.b += P b,
@@ -351,36 +426,123 @@ int op_hmult(const Context &c, int level, uint32_t *out_b, uint32_t *out_a,
 int op_mult_key(const Context &c, int level, uint32_t *out_b, uint32_t *out_a,
                 const uint32_t *b, const uint32_t *a,
                 const uint32_t *const *key_b, const uint32_t *const *key_a,
-                uint32_t *ws, hipStream_t st) {
+                uint32_t *ws, Sink &s) {
   const LevelPlan &lp = c.levels[level];
   const int tot = lp.num_q + c.alpha;
   uint32_t *acc_b = ws;
   uint32_t *acc_a = ws + (size_t)tot * kLimb;
   uint32_t *rest = ws + 2 * (size_t)tot * kLimb;
   int rc = op_mult_key_no_moddown(c, level, acc_b, acc_a, a, b, nullptr, key_b,
-                                  key_a, rest, st);
+                                  key_a, rest, s);
   if (rc) return rc;
   uint32_t *dst[2] = {out_b, out_a};
   const uint32_t *src[2] = {acc_b, acc_a};
-  return op_moddown(c, level, 0, 2, dst, src, rest, st);
+  return op_moddown(c, level, 0, 2, dst, src, rest, s);
 }
 
 // HRot / HConj (Context.cpp:776-798): MultKey then Permute by galois factor
 int op_hrot_galois(const Context &c, int level, uint32_t *out_b,
                    uint32_t *out_a, const uint32_t *b, const uint32_t *a,
                    const uint32_t *const *key_b, const uint32_t *const *key_a,
-                   uint32_t galois, hipStream_t st) {
+                   uint32_t galois, uint32_t *ws, Sink &s) {
   const LevelPlan &lp = c.levels[level];
   const size_t qs = (size_t)lp.num_q * kLimb;
-  uint32_t *tmp = c.ws;
+  uint32_t *tmp = ws;
   int rc = op_mult_key(c, level, tmp, tmp + qs, b, a, key_b, key_a,
-                       tmp + 2 * qs, st);
+                       tmp + 2 * qs, s);
   if (rc) return rc;
   EwNp np = make_np(c, lp.nm, lp.nt, 0);
   EwOperand x{{tmp, tmp + qs, nullptr}, 0};
   EwOperand o{{out_b, out_a, nullptr}, 0};
-  launch_permute(c, np, x, o, galois, 2, st);
-  return launch_status("permute");
+  return s.permute(np, x, o, galois, 2);
+}
+
+// ---------------------------------------------------------------------------
+// HoistHandler: hoisted rotations
+// ---------------------------------------------------------------------------
+
+size_t hoisted_ws_limbs(const Context &c, int level, int n_rot, bool moddown) {
+  const LevelPlan &lp = c.levels[level];
+  const size_t tot = (size_t)lp.num_q + c.alpha;
+  size_t need = mult_key_ws_limbs(c, level);
+  if (moddown) {
+    const int r = std::min(n_rot, kMaxBsRot);
+    need += 2 * (size_t)r * tot + 2 * (size_t)r * c.alpha;
+  }
+  return need;
+}
+
+// EvaluateBabyStep (Hoist.cu:578-691) with the fused key product
+// (BSFusedKeyMult :207-327), followed, when `moddown`, by the ModDown of
+// every baby-step output (the MultKey tail of HRot): the results equal
+// HRot(ct, rots[i]) bit for bit, because ModDown commutes with the
+// automorphism and ModDown(P b + s) = b + ModDown(s).
+int op_hoisted_rotations(const Context &c, int level, int n_rot,
+                         const int *rots, uint32_t *const *out_b,
+                         uint32_t *const *out_a, const uint32_t *b,
+                         const uint32_t *a, const uint32_t *const *key_b,
+                         const uint32_t *const *key_a, bool moddown,
+                         uint32_t *ws, Sink &s) {
+  const LevelPlan &lp = c.levels[level];
+  const int tot = lp.num_q + c.alpha;
+  uint32_t *v = ws;
+  uint32_t *modup = ws + (size_t)lp.num_q * kLimb;
+  uint32_t *acc = modup + (size_t)lp.beta * tot * kLimb;
+  int rc = op_modup(c, level, modup, a, v, s, false);
+  if (rc) return rc;
+  BsFusedArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.modup = modup;
+  A.x = a;
+  A.b = b;
+  A.p_prod = c.d_p_prod;
+  for (int g = 0; g < lp.beta; g++) {
+    if (lp.groups[g].s1 <= lp.groups[g].s0) continue;
+    if (A.ngroups == kMaxBsGroups) {
+      set_error("hoisted rotations support at most 8 key groups");
+      return kErrUnsupported;
+    }
+    A.group[A.ngroups] = g;
+    A.own0[A.ngroups] = lp.groups[g].s0;
+    A.own1[A.ngroups] = lp.groups[g].s1;
+    A.ngroups++;
+  }
+  const EwNp np = make_np(c, lp.nm, lp.nt, c.alpha);
+  for (int r0 = 0; r0 < n_rot; r0 += kMaxBsRot) {
+    const int nr = std::min(kMaxBsRot, n_rot - r0);
+    A.nrot = nr;
+    std::vector<uint32_t *> md_dst, md_src;
+    for (int r = 0; r < nr; r++) {
+      const int rot = rots[r0 + r];
+      // scatter by g^-1 = 5^(N/2 - rot) (Hoist.cu:300-302)
+      A.ginv[r] = c.galois[c.n / 2 - rot];
+      for (int gi = 0; gi < A.ngroups; gi++) {
+        const int g = A.group[gi];
+        A.key_b[r][gi] = key_b[(size_t)(r0 + r) * c.dnum + g];
+        A.key_a[r][gi] = key_a[(size_t)(r0 + r) * c.dnum + g];
+      }
+      if (moddown) {
+        A.out_b[r] = acc + (size_t)(2 * r) * tot * kLimb;
+        A.out_a[r] = acc + (size_t)(2 * r + 1) * tot * kLimb;
+        md_src.push_back(A.out_b[r]);
+        md_src.push_back(A.out_a[r]);
+        md_dst.push_back(out_b[r0 + r]);
+        md_dst.push_back(out_a[r0 + r]);
+      } else {
+        A.out_b[r] = out_b[r0 + r];
+        A.out_a[r] = out_a[r0 + r];
+      }
+    }
+    rc = s.bs_fused(np, A);
+    if (rc) return rc;
+    if (moddown) {
+      uint32_t *mv = acc + (size_t)(2 * nr) * tot * kLimb;
+      std::vector<const uint32_t *> csrc(md_src.begin(), md_src.end());
+      rc = op_moddown(c, level, 0, 2 * nr, md_dst.data(), csrc.data(), mv, s);
+      if (rc) return rc;
+    }
+  }
+  return kOk;
 }
 
 }  // namespace ck

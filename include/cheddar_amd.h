@@ -206,6 +206,20 @@ int ck_hconj(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
 int ck_rescale(ck_context *ctx, int level, int npoly, uint32_t *const *dst,
                const uint32_t *const *src, void *stream);
 
+/* Hoisted rotations: HoistHandler::EvaluateBabyStep with the fused key
+ * product (src/extension/Hoist.cu:578-691, BSFusedKeyMult :207-327,
+ * BSFusedKernel :21-78). One ModUp of a serves n_rot rotations by
+ * rots[i] in [1, N/2); key_b / key_a are n_rot x dnum device pointers
+ * (row i: the rotation key of rots[i]). moddown != 0: out_*[i] =
+ * HRot(ct, rots[i]) (num_q limbs, identical to ck_hrot); moddown == 0:
+ * the baby-step output in the Q*P basis (num_q + alpha limbs), as the
+ * reference keeps it for the giant step. */
+int ck_hrot_hoisted(ck_context *ctx, int level, int n_rot, const int *rots,
+                    uint32_t *const *out_b, uint32_t *const *out_a,
+                    const uint32_t *b, const uint32_t *a,
+                    const uint32_t *const *key_b, const uint32_t *const *key_a,
+                    int moddown, void *stream);
+
 /* ---- Per-kernel timing ---------------------------------------------------- */
 /* Replaces the wall-clock profiling macros of the reference's test harness
  * (unittest/Testbed.h:24-47) with device-side timing: while enabled, every

@@ -1057,3 +1057,36 @@ int or_hrot(const or_param *p, int level, const uint32_t *b,
   free(tmp);
   return rc;
 }
+
+/* HoistHandler baby step for one rotation: MultKeyNoModDown(a) over the
+ * key groups plus P * b on the q limbs (PseudoModUp, Hoist.cu:627-651), then
+ * BSFusedKernel's scatter (Hoist.cu:41-75): coefficient x of every limb goes
+ * to brv17((brv17(x) + 1) * g - 1) with g = 5^(N/2 - rot) (Hoist.cu:300-302).
+ * The result stays in the Q*P basis. */
+int or_hoisted_baby_step(const or_param *p, int level, const uint32_t *b,
+                         const uint32_t *a, const uint32_t *rk_b,
+                         const uint32_t *rk_a, int rot, uint32_t *out_b,
+                         uint32_t *out_a) {
+  const int n = p->n, alpha = p->alpha, lb = p->log_n + 1;
+  const uint32_t mask = (2u << p->log_n) - 1;
+  if (level < 0 || level > p->max_level || rot < 1 || rot >= n / 2) return -1;
+  int nm = p->lv_main[level], nt = p->lv_ter[level];
+  int tot = nm + nt + alpha;
+  uint32_t *acc = (uint32_t *)malloc(sizeof(uint32_t) * 2 * (size_t)tot * n);
+  int rc = or_mult_key_no_moddown(p, level, a, b, NULL, rk_b, rk_a, acc,
+                                  acc + (size_t)tot * n);
+  if (rc == 0) {
+    const uint32_t g = or_galois_factor(p, n / 2 - rot);
+#pragma omp parallel for schedule(static) num_threads(g_num_threads)
+    for (int t = 0; t < tot; t++) {
+      for (int x = 0; x < n; x++) {
+        uint32_t e = or_bitrev((uint32_t)x, lb) + 1;
+        uint32_t d = or_bitrev(((e * g) - 1) & mask, lb);
+        out_b[(size_t)t * n + d] = acc[(size_t)t * n + x];
+        out_a[(size_t)t * n + d] = acc[(size_t)(tot + t) * n + x];
+      }
+    }
+  }
+  free(acc);
+  return rc;
+}

@@ -82,6 +82,8 @@ def _load():
         "or_mult_key_no_moddown": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [_u32p] * 7),
         "or_hmult": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [_u32p] * 6 + [ctypes.c_int, _u32p, _u32p]),
         "or_hrot": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [_u32p] * 4 + [ctypes.c_int, _u32p, _u32p]),
+        "or_hoisted_baby_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [_u32p] * 4 +
+                                 [ctypes.c_int, _u32p, _u32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -324,6 +326,17 @@ class Param:
         rc = lib.or_hmult(self._h, level, *[_p(a) for a in ins], int(rescale), _p(ob), _p(oa))
         if rc != 0:
             raise ValueError("or_hmult failed")
+        return ob, oa
+
+    def hoisted_baby_step(self, level, ct, rk, rot):
+        tot = self.num_q(level) + self.alpha
+        ob = np.empty((tot, self.n), dtype=np.uint32)
+        oa = np.empty((tot, self.n), dtype=np.uint32)
+        ins = [np.ascontiguousarray(v, dtype=np.uint32) for v in (ct[0], ct[1], rk[0], rk[1])]
+        rc = lib.or_hoisted_baby_step(self._h, level, *[_p(a) for a in ins], int(rot),
+                                      _p(ob), _p(oa))
+        if rc != 0:
+            raise ValueError("or_hoisted_baby_step failed")
         return ob, oa
 
     def hrot(self, level, ct, rk, rot):

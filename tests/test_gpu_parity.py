@@ -310,3 +310,67 @@ def test_boot40_hmult(ck, ctx40, oracle_boot40, level):
     sync()
     assert_same(ck.to_host(out[0]), want[0], "boot40 hmult b")
     assert_same(ck.to_host(out[1]), want[1], "boot40 hmult a")
+
+
+# ---------------------------------------------------------------------------
+# Hoisted rotations (HoistHandler baby step, src/extension/Hoist.cu)
+# ---------------------------------------------------------------------------
+def _rot_keys(P, n, seed):
+    return [full_key(P, seed + 16 * i) for i in range(n)]
+
+
+@pytest.mark.parametrize("level", [23, 7])
+def test_hoisted_baby_step(ck, ctx24, oracle_rns24, level):
+    P = oracle_rns24
+    pidx = P.level_primes(level)
+    ct = [uniform(P, pidx, SEED_BASE + 140 + i) for i in range(2)]
+    rots = [1, 3, 1234, 32767]
+    keys = _rot_keys(P, len(rots), SEED_BASE + 150)
+    tot = P.num_q(level) + P.alpha
+    outs = [[torch.empty((tot, P.n), dtype=torch.int32, device="cuda") for _ in range(2)]
+            for _ in rots]
+    ctx24.hrot_hoisted(outs, [ck.to_device(v) for v in ct], [dev_key(*k) for k in keys],
+                       rots, level, moddown=False)
+    sync()
+    for r, k, o in zip(rots, keys, outs):
+        want = P.hoisted_baby_step(level, ct, k, r)
+        assert_same(ck.to_host(o[0]), want[0], f"bs b rot {r}")
+        assert_same(ck.to_host(o[1]), want[1], f"bs a rot {r}")
+
+
+def test_hoisted_rotations_equal_hrot(ck, ctx24, oracle_rns24):
+    """10 rotations (two launches of the fused kernel) + ModDown == HRot"""
+    P = oracle_rns24
+    level = 15
+    pidx = P.level_primes(level)
+    ct = [uniform(P, pidx, SEED_BASE + 160 + i) for i in range(2)]
+    rots = [1, 2, 5, 77, 1000, 4096, 8191, 16384, 30000, 32767]
+    keys = _rot_keys(P, len(rots), SEED_BASE + 170)
+    outs = [[torch.empty((P.num_q(level), P.n), dtype=torch.int32, device="cuda")
+             for _ in range(2)] for _ in rots]
+    ctx24.hrot_hoisted(outs, [ck.to_device(v) for v in ct], [dev_key(*k) for k in keys],
+                       rots, level, moddown=True)
+    sync()
+    for r, k, o in zip(rots, keys, outs):
+        want = P.hrot(level, ct, k, r)
+        assert_same(ck.to_host(o[0]), want[0], f"hoisted hrot b rot {r}")
+        assert_same(ck.to_host(o[1]), want[1], f"hoisted hrot a rot {r}")
+
+
+def test_boot40_hoisted_rotations(ck, ctx40, oracle_boot40):
+    """terminal primes and dnum 4: hoisted HRot == HRot"""
+    P = oracle_boot40
+    level = 20
+    pidx = P.level_primes(level)
+    ct = [uniform(P, pidx, SEED_BASE + 180 + i) for i in range(2)]
+    rots = [3, 12345]
+    keys = _rot_keys(P, len(rots), SEED_BASE + 190)
+    outs = [[torch.empty((P.num_q(level), P.n), dtype=torch.int32, device="cuda")
+             for _ in range(2)] for _ in rots]
+    ctx40.hrot_hoisted(outs, [ck.to_device(v) for v in ct], [dev_key(*k) for k in keys],
+                       rots, level, moddown=True)
+    sync()
+    for r, k, o in zip(rots, keys, outs):
+        want = P.hrot(level, ct, k, r)
+        assert_same(ck.to_host(o[0]), want[0], f"boot40 hoisted b rot {r}")
+        assert_same(ck.to_host(o[1]), want[1], f"boot40 hoisted a rot {r}")

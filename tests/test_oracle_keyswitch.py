@@ -135,3 +135,36 @@ def test_boot40_hmult_decrypts(oracle_boot40):
     ks = [0, 17, P.n - 1]
     for k, v in zip(ks, K.decrypt(out, level - 1, ks)):
         assert abs(v - round(prod[k] / D)) < 2 ** 16, k
+
+
+@pytest.mark.parametrize("rot", [1, 1234, 32767])
+def test_hoisted_baby_step_is_permuted_key_product(oracle_rns24, rot):
+    """BSFusedKernel's scatter by 5^(N/2 - r) (Hoist.cu:21-78) is the
+    Permute gather by 5^r of MultKeyNoModDown(a) + P b, on all Q*P limbs"""
+    from gpu_util import full_key
+    P = oracle_rns24
+    level = 9
+    pidx = P.level_primes(level)
+    ct = [P.uniform(pidx, SEED_BASE + 700 + i) for i in range(2)]
+    key = full_key(P, SEED_BASE + 710)
+    bs = P.hoisted_baby_step(level, ct, key, rot)
+    acc = P.mult_key_no_moddown(level, ct[1], key[0], key[1], add_b=ct[0])
+    g = P.galois(rot)
+    assert np.array_equal(bs[0], P.permute(acc[0], g))
+    assert np.array_equal(bs[1], P.permute(acc[1], g))
+
+
+@pytest.mark.parametrize("level,rot", [(23, 5), (4, 20000)])
+def test_hoisted_then_moddown_equals_hrot(oracle_rns24, level, rot):
+    """ModDown commutes with the automorphism and ModDown(P b + s) =
+    b + ModDown(s): the hoisted baby step followed by ModDown is HRot"""
+    from gpu_util import full_key
+    P = oracle_rns24
+    pidx = P.level_primes(level)
+    ct = [P.uniform(pidx, SEED_BASE + 720 + i) for i in range(2)]
+    key = full_key(P, SEED_BASE + 730)
+    bs = P.hoisted_baby_step(level, ct, key, rot)
+    got = [P.moddown(v, level, O.OR_MODDOWN) for v in bs]
+    want = P.hrot(level, ct, key, rot)
+    assert np.array_equal(got[0], want[0])
+    assert np.array_equal(got[1], want[1])
