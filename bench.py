@@ -218,6 +218,84 @@ def cpu_baseline(prm, seconds=10.0):
                       f"({el:.1f} s)"}
 
 
+BATCH = 64
+BATCH_ROTS = 16
+
+
+def bench_batched(a, torch, ck):
+    """Batched key switching: BATCH ciphertexts with limb counts uniform in
+    [8, 24] (seeded) each go through HMult+relin+rescale (one plan, one
+    launch per step across the batch), then BATCH_ROTS hoisted rotations by
+    distinct offsets in [1, 2^15) of each product (a second plan sharing one
+    set of rotation keys). Steps = runs of both plans; HMult/s and HRot/s
+    are whole-batch throughputs."""
+    import random
+    prm = ck.load_param_json(PARAMS)
+    ctx = ck.Context.from_json(prm)
+    torch.cuda.set_device(0)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0x4348454444415202)
+    rng = random.Random(0x4348454444415202)
+    levels = [rng.randint(8, 24) - 1 for _ in range(BATCH)]
+    rots = rng.sample(range(1, 1 << 15), BATCH_ROTS)
+    allp = list(range(ctx.num_primes))
+
+    def qprimes(level):
+        nm, nt = ctx.level_np(level)
+        return list(range(ctx.max_ter - nt, ctx.max_ter + nm))
+
+    def key():
+        return ([uniform_poly(torch, ctx, allp, gen) for _ in range(ctx.dnum)],
+                [uniform_poly(torch, ctx, allp, gen) for _ in range(ctx.dnum)])
+
+    ct1 = [[uniform_poly(torch, ctx, qprimes(l), gen) for _ in range(2)] for l in levels]
+    ct2 = [[uniform_poly(torch, ctx, qprimes(l), gen) for _ in range(2)] for l in levels]
+    prod = [[torch.empty((ctx.num_q(l - 1), ctx.n), dtype=torch.int32, device="cuda")
+             for _ in range(2)] for l in levels]
+    rot_out = [[[torch.empty((ctx.num_q(l - 1), ctx.n), dtype=torch.int32, device="cuda")
+                 for _ in range(2)] for _ in rots] for l in levels]
+    mkey = key()
+    rkeys = [key() for _ in rots]
+    t0 = time.perf_counter()
+    p_mult = ck.Plan.hmult(ctx, levels, prod, ct1, ct2, mkey)
+    p_rot = ck.Plan.hrot_hoisted(ctx, [l - 1 for l in levels], prod, rkeys, rots, rot_out)
+    log(f"plans built in {time.perf_counter() - t0:.1f} s: hmult {p_mult.info()}, "
+        f"hoisted {p_rot.info()}")
+    stream = torch.cuda.current_stream()
+    for _ in range(a.warmup):
+        p_mult.run(stream)
+        p_rot.run(stream)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record(stream)
+    for _ in range(a.steps):
+        p_mult.run(stream)
+    ev[1].record(stream)
+    for _ in range(a.steps):
+        p_rot.run(stream)
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    t_mult = ev[0].elapsed_time(ev[1]) / a.steps
+    t_rot = ev[1].elapsed_time(ev[2]) / a.steps
+    hmult_s = BATCH / (t_mult * 1e-3)
+    hrot_s = BATCH * BATCH_ROTS / (t_rot * 1e-3)
+    log(f"batched HMult: {t_mult:.3f} ms per {BATCH} ({hmult_s:.0f}/s); hoisted HRot: "
+        f"{t_rot:.3f} ms per {BATCH * BATCH_ROTS} ({hrot_s:.0f}/s)")
+    res = {
+        "metric": "hmult_relin_rescale_per_s", "value": hmult_s, "unit": "HMult/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": t_mult + t_rot, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic uniform residues and keys (seeded)",
+        "config": {"workload": f"batched: {BATCH} ciphertexts at 8..24 limbs, HMult+relin+"
+                               f"rescale plan then {BATCH_ROTS} hoisted HRot each",
+                   "levels": levels, "rotations": rots, "log_n": 16, "alpha": ctx.alpha,
+                   "dnum": ctx.dnum, "parallelism": "single GPU"},
+        "hrot_per_s": hrot_s, "ms_hmult_batch": t_mult, "ms_hrot_batch": t_rot,
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,6 +308,10 @@ def main():
     ap.add_argument("--rocprof-summary", metavar="KERNEL_TRACE_CSV",
                     help="print a per-kernel summary of a rocprofv3 --kernel-trace "
                          "CSV and exit")
+    ap.add_argument("--workload", choices=("hmult", "batched"), default="hmult",
+                    help="hmult: one HMult+relin+rescale per step at L=24 (default); "
+                         "batched: 64 ciphertexts at 8..24 limbs through one HMult "
+                         "plan, then 16 hoisted rotations each through one plan")
     ap.add_argument("--pmc-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print per-kernel HBM traffic (JSON) from two rocprofv3 --pmc "
                          "runs (FETCH_SIZE, WRITE_SIZE) and exit")
@@ -247,6 +329,9 @@ def main():
 
     import torch
     import cheddar_amd as ck
+
+    if a.workload == "batched":
+        return bench_batched(a, torch, ck)
 
     prm = ck.load_param_json(PARAMS)
     ctx = ck.Context.from_json(prm)

@@ -74,6 +74,16 @@ _SIGS = {
                                        ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, _vp,
                                        ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int,
                                        _vp]),
+    "ck_plan_hmult": (ctypes.c_int, [_vp, ctypes.c_int, _ip, ctypes.c_int] +
+                      [ctypes.POINTER(_vp)] * 8 + [ctypes.POINTER(_vp)]),
+    "ck_plan_hrot_hoisted": (ctypes.c_int, [_vp, ctypes.c_int, _ip, ctypes.POINTER(_vp),
+                                            ctypes.POINTER(_vp), ctypes.c_int, _ip,
+                                            ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                            ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                            ctypes.c_int, ctypes.POINTER(_vp)]),
+    "ck_plan_run": (ctypes.c_int, [_vp, _vp]),
+    "ck_plan_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
+    "ck_plan_destroy": (ctypes.c_int, [_vp]),
     "ck_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ck_profile_report": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p,
                                          ctypes.POINTER(ctypes.c_double),
@@ -327,6 +337,57 @@ class Context:
             name = raw[64 * i:64 * (i + 1)].split(b"\0", 1)[0].decode()
             out[name] = (ms[i], calls[i])
         return out
+
+
+class Plan:
+    """A batched plan (ck_plan_*): built once over fixed device buffers, run
+    any number of times; keeps references to the tensors it was built on."""
+
+    def __init__(self, ctx, handle, keep):
+        self._ctx = ctx
+        self._h = handle
+        self._keep = keep
+
+    @classmethod
+    def hmult(cls, ctx, levels, outs, ct1s, ct2s, key, rescale=True):
+        n = len(levels)
+        h = ctypes.c_void_p()
+        _check(lib.ck_plan_hmult(ctx._h, n, (ctypes.c_int * n)(*levels), int(rescale),
+                                 _ptrs([o[0] for o in outs]), _ptrs([o[1] for o in outs]),
+                                 _ptrs([c[0] for c in ct1s]), _ptrs([c[1] for c in ct1s]),
+                                 _ptrs([c[0] for c in ct2s]), _ptrs([c[1] for c in ct2s]),
+                                 _ptrs(key[0]), _ptrs(key[1]), ctypes.byref(h)))
+        return cls(ctx, h, (outs, ct1s, ct2s, key))
+
+    @classmethod
+    def hrot_hoisted(cls, ctx, levels, cts, keys, rots, outs, moddown=True):
+        """outs[i][r] = (b, a) output of rotation rots[r] of cts[i]"""
+        n, nr = len(levels), len(rots)
+        kb = [t for k in keys for t in k[0]]
+        ka = [t for k in keys for t in k[1]]
+        flat = [o for row in outs for o in row]
+        h = ctypes.c_void_p()
+        _check(lib.ck_plan_hrot_hoisted(ctx._h, n, (ctypes.c_int * n)(*levels),
+                                        _ptrs([c[0] for c in cts]), _ptrs([c[1] for c in cts]),
+                                        nr, (ctypes.c_int * nr)(*rots), _ptrs(kb), _ptrs(ka),
+                                        _ptrs([o[0] for o in flat]), _ptrs([o[1] for o in flat]),
+                                        int(moddown), ctypes.byref(h)))
+        return cls(ctx, h, (cts, keys, outs))
+
+    def run(self, stream=None):
+        _check(lib.ck_plan_run(self._h, _stream(stream)))
+
+    def info(self):
+        out = (ctypes.c_longlong * 4)()
+        _check(lib.ck_plan_info(self._h, out))
+        return {"items": out[0], "steps": out[1], "launches": out[2],
+                "workspace_bytes": out[3]}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and lib is not None:
+            lib.ck_plan_destroy(h)
+            self._h = None
 
 
 # ---- host <-> device helpers ------------------------------------------------

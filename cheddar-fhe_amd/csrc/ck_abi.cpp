@@ -4,11 +4,16 @@
 #include "../../include/cheddar_amd.h"
 
 #include <cstring>
+#include <vector>
 
-#include "ck_ops.h"
+#include "ck_plan.h"
 
 struct ck_context {
   ck::Context *c;
+};
+
+struct ck_plan {
+  ck::Plan *p;
 };
 
 using namespace ck;
@@ -443,11 +448,127 @@ int ck_hrot_hoisted(ck_context *ctx, int level, int n_rot, const int *rots,
     CK_REQUIRE(out_b[i] != b && out_b[i] != a && out_a[i] != b && out_a[i] != a,
                "in-place operation is not supported for hoisted rotations");
   }
-  CK_REQUIRE(hoisted_ws_limbs(c, level, n_rot, moddown != 0) * kLimb <= c.ws_words,
+  CK_REQUIRE(hoisted_ws_limbs(c, level, kMaxBsRot, moddown != 0) * kLimb <=
+                 c.ws_words,
              "workspace too small");
   ImmediateSink sink(c, as_stream(stream));
   return op_hoisted_rotations(c, level, n_rot, rots, out_b, out_a, b, a, key_b,
-                              key_a, moddown != 0, c.ws, sink);
+                              key_a, moddown != 0, kMaxBsRot, c.ws, sink);
+}
+
+int ck_plan_hmult(ck_context *ctx, int n, const int *levels, int rescale,
+                  uint32_t *const *out_b, uint32_t *const *out_a,
+                  const uint32_t *const *b1, const uint32_t *const *a1,
+                  const uint32_t *const *b2, const uint32_t *const *a2,
+                  const uint32_t *const *key_b, const uint32_t *const *key_a,
+                  ck_plan **out) {
+  CK_REQUIRE(out, "out is null");
+  *out = nullptr;
+  CK_REQUIRE(ctx && ctx->c && levels, "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(n >= 1, "a plan needs at least one ciphertext");
+  CK_REQUIRE(valid_ptrs(out_b, n) && valid_ptrs(out_a, n) && valid_ptrs(b1, n) &&
+                 valid_ptrs(a1, n) && valid_ptrs(b2, n) && valid_ptrs(a2, n),
+             "null pointer");
+  size_t total = 0;
+  std::vector<size_t> off(n);
+  for (int i = 0; i < n; i++) {
+    CK_REQUIRE(valid_level(c, levels[i]), "invalid level");
+    CK_REQUIRE(!rescale || moddown_plan(c, levels[i], 2),
+               "cannot rescale at this level");
+    CK_REQUIRE(valid_key(c, key_b, levels[i]) && valid_key(c, key_a, levels[i]),
+               "null key block");
+    off[i] = total;
+    total += relin_ws_limbs(c, levels[i]);
+  }
+  Plan *p = new Plan;
+  p->ctx = &c;
+  p->items = n;
+  uint32_t *ws = nullptr;
+  int rc = plan_alloc_workspace(*p, total, &ws);
+  PlanSink sink(*p);
+  for (int i = 0; i < n && !rc; i++) {
+    sink.begin_item();
+    rc = op_hmult(c, levels[i], out_b[i], out_a[i], b1[i], a1[i], b2[i], a2[i],
+                  key_b, key_a, rescale != 0, ws + off[i] * kLimb, sink);
+  }
+  if (!rc) rc = plan_finish(*p, sink.mismatch());
+  if (rc) {
+    plan_destroy(p);
+    return rc;
+  }
+  *out = new ck_plan{p};
+  return CK_OK;
+}
+
+int ck_plan_hrot_hoisted(ck_context *ctx, int n, const int *levels,
+                         const uint32_t *const *b, const uint32_t *const *a,
+                         int n_rot, const int *rots,
+                         const uint32_t *const *key_b,
+                         const uint32_t *const *key_a, uint32_t *const *out_b,
+                         uint32_t *const *out_a, int moddown, ck_plan **out) {
+  CK_REQUIRE(out, "out is null");
+  *out = nullptr;
+  CK_REQUIRE(ctx && ctx->c && levels && rots && key_b && key_a, "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(n >= 1 && n_rot >= 1, "a plan needs ciphertexts and rotations");
+  CK_REQUIRE(valid_ptrs(b, n) && valid_ptrs(a, n), "null input pointer");
+  CK_REQUIRE(valid_ptrs(out_b, n * n_rot) && valid_ptrs(out_a, n * n_rot),
+             "null output pointer");
+  size_t total = 0;
+  std::vector<size_t> off(n);
+  for (int i = 0; i < n; i++) {
+    CK_REQUIRE(valid_level(c, levels[i]), "invalid level");
+    for (int r = 0; r < n_rot; r++)
+      CK_REQUIRE(valid_key(c, key_b + (size_t)r * c.dnum, levels[i]) &&
+                     valid_key(c, key_a + (size_t)r * c.dnum, levels[i]),
+                 "null key block");
+    off[i] = total;
+    total += hoisted_ws_limbs(c, levels[i], n_rot, moddown != 0);
+  }
+  for (int r = 0; r < n_rot; r++)
+    CK_REQUIRE(rots[r] >= 1 && rots[r] < c.n / 2,
+               "hoisted rotation amounts must be in [1, N/2)");
+  Plan *p = new Plan;
+  p->ctx = &c;
+  p->items = n;
+  uint32_t *ws = nullptr;
+  int rc = plan_alloc_workspace(*p, total, &ws);
+  PlanSink sink(*p);
+  for (int i = 0; i < n && !rc; i++) {
+    sink.begin_item();
+    rc = op_hoisted_rotations(c, levels[i], n_rot, rots, out_b + (size_t)i * n_rot,
+                              out_a + (size_t)i * n_rot, b[i], a[i], key_b, key_a,
+                              moddown != 0, n_rot, ws + off[i] * kLimb, sink);
+  }
+  if (!rc) rc = plan_finish(*p, sink.mismatch());
+  if (rc) {
+    plan_destroy(p);
+    return rc;
+  }
+  *out = new ck_plan{p};
+  return CK_OK;
+}
+
+int ck_plan_run(ck_plan *plan, void *stream) {
+  CK_REQUIRE(plan && plan->p, "null plan");
+  return plan_run(*plan->p, as_stream(stream));
+}
+
+int ck_plan_info(const ck_plan *plan, long long *out) {
+  CK_REQUIRE(plan && plan->p && out, "null argument");
+  out[0] = plan->p->items;
+  out[1] = (long long)plan->p->stages.size();
+  out[2] = plan_launches(*plan->p);
+  out[3] = (long long)plan->p->workspace_bytes;
+  return CK_OK;
+}
+
+int ck_plan_destroy(ck_plan *plan) {
+  if (!plan) return CK_OK;
+  plan_destroy(plan->p);
+  delete plan;
+  return CK_OK;
 }
 
 int ck_profile_enable(ck_context *ctx, int on) {

@@ -461,14 +461,12 @@ int op_hrot_galois(const Context &c, int level, uint32_t *out_b,
 // HoistHandler: hoisted rotations
 // ---------------------------------------------------------------------------
 
-size_t hoisted_ws_limbs(const Context &c, int level, int n_rot, bool moddown) {
+size_t hoisted_ws_limbs(const Context &c, int level, int chunk_rot,
+                        bool moddown) {
   const LevelPlan &lp = c.levels[level];
   const size_t tot = (size_t)lp.num_q + c.alpha;
   size_t need = mult_key_ws_limbs(c, level);
-  if (moddown) {
-    const int r = std::min(n_rot, kMaxBsRot);
-    need += 2 * (size_t)r * tot + 2 * (size_t)r * c.alpha;
-  }
+  if (moddown) need += 2 * (size_t)chunk_rot * (tot + c.alpha);
   return need;
 }
 
@@ -476,13 +474,15 @@ size_t hoisted_ws_limbs(const Context &c, int level, int n_rot, bool moddown) {
 // (BSFusedKeyMult :207-327), followed, when `moddown`, by the ModDown of
 // every baby-step output (the MultKey tail of HRot): the results equal
 // HRot(ct, rots[i]) bit for bit, because ModDown commutes with the
-// automorphism and ModDown(P b + s) = b + ModDown(s).
+// automorphism and ModDown(P b + s) = b + ModDown(s). The workspace holds
+// the accumulators of chunk_rot rotations; each chunk's outputs share one
+// ModDown.
 int op_hoisted_rotations(const Context &c, int level, int n_rot,
                          const int *rots, uint32_t *const *out_b,
                          uint32_t *const *out_a, const uint32_t *b,
                          const uint32_t *a, const uint32_t *const *key_b,
                          const uint32_t *const *key_a, bool moddown,
-                         uint32_t *ws, Sink &s) {
+                         int chunk_rot, uint32_t *ws, Sink &s) {
   const LevelPlan &lp = c.levels[level];
   const int tot = lp.num_q + c.alpha;
   uint32_t *v = ws;
@@ -508,37 +508,41 @@ int op_hoisted_rotations(const Context &c, int level, int n_rot,
     A.ngroups++;
   }
   const EwNp np = make_np(c, lp.nm, lp.nt, c.alpha);
-  for (int r0 = 0; r0 < n_rot; r0 += kMaxBsRot) {
-    const int nr = std::min(kMaxBsRot, n_rot - r0);
-    A.nrot = nr;
-    std::vector<uint32_t *> md_dst, md_src;
-    for (int r = 0; r < nr; r++) {
-      const int rot = rots[r0 + r];
-      // scatter by g^-1 = 5^(N/2 - rot) (Hoist.cu:300-302)
-      A.ginv[r] = c.galois[c.n / 2 - rot];
-      for (int gi = 0; gi < A.ngroups; gi++) {
-        const int g = A.group[gi];
-        A.key_b[r][gi] = key_b[(size_t)(r0 + r) * c.dnum + g];
-        A.key_a[r][gi] = key_a[(size_t)(r0 + r) * c.dnum + g];
+  if (chunk_rot < 1) chunk_rot = 1;
+  for (int c0 = 0; c0 < n_rot; c0 += chunk_rot) {
+    const int nc = std::min(chunk_rot, n_rot - c0);
+    std::vector<uint32_t *> md_dst;
+    std::vector<const uint32_t *> md_src;
+    for (int r0 = c0; r0 < c0 + nc; r0 += kMaxBsRot) {
+      const int nr = std::min(kMaxBsRot, c0 + nc - r0);
+      A.nrot = nr;
+      for (int r = 0; r < nr; r++) {
+        const int i = r0 + r;
+        // scatter by g^-1 = 5^(N/2 - rot) (Hoist.cu:300-302)
+        A.ginv[r] = c.galois[c.n / 2 - rots[i]];
+        for (int gi = 0; gi < A.ngroups; gi++) {
+          const int g = A.group[gi];
+          A.key_b[r][gi] = key_b[(size_t)i * c.dnum + g];
+          A.key_a[r][gi] = key_a[(size_t)i * c.dnum + g];
+        }
+        if (moddown) {
+          A.out_b[r] = acc + (size_t)(2 * (i - c0)) * tot * kLimb;
+          A.out_a[r] = acc + (size_t)(2 * (i - c0) + 1) * tot * kLimb;
+          md_src.push_back(A.out_b[r]);
+          md_src.push_back(A.out_a[r]);
+          md_dst.push_back(out_b[i]);
+          md_dst.push_back(out_a[i]);
+        } else {
+          A.out_b[r] = out_b[i];
+          A.out_a[r] = out_a[i];
+        }
       }
-      if (moddown) {
-        A.out_b[r] = acc + (size_t)(2 * r) * tot * kLimb;
-        A.out_a[r] = acc + (size_t)(2 * r + 1) * tot * kLimb;
-        md_src.push_back(A.out_b[r]);
-        md_src.push_back(A.out_a[r]);
-        md_dst.push_back(out_b[r0 + r]);
-        md_dst.push_back(out_a[r0 + r]);
-      } else {
-        A.out_b[r] = out_b[r0 + r];
-        A.out_a[r] = out_a[r0 + r];
-      }
+      rc = s.bs_fused(np, A);
+      if (rc) return rc;
     }
-    rc = s.bs_fused(np, A);
-    if (rc) return rc;
     if (moddown) {
-      uint32_t *mv = acc + (size_t)(2 * nr) * tot * kLimb;
-      std::vector<const uint32_t *> csrc(md_src.begin(), md_src.end());
-      rc = op_moddown(c, level, 0, 2 * nr, md_dst.data(), csrc.data(), mv, s);
+      uint32_t *mv = acc + (size_t)(2 * nc) * tot * kLimb;
+      rc = op_moddown(c, level, 0, 2 * nc, md_dst.data(), md_src.data(), mv, s);
       if (rc) return rc;
     }
   }

@@ -102,13 +102,15 @@ int op_hrot_galois(const Context &c, int level, uint32_t *out_b,
 // one ModUp of a shared by n_rot rotations. key_b / key_a hold n_rot rows of
 // dnum key blocks; rots[i] in [1, N/2). With moddown, out_*[i] is
 // HRot(ct, rots[i]) (num_q limbs); without, the Q*P-basis baby-step output
-// (num_q + alpha limbs).
-size_t hoisted_ws_limbs(const Context &c, int level, int n_rot, bool moddown);
+// (num_q + alpha limbs). The workspace (hoisted_ws_limbs) holds the
+// accumulators of chunk_rot rotations at a time.
+size_t hoisted_ws_limbs(const Context &c, int level, int chunk_rot,
+                        bool moddown);
 int op_hoisted_rotations(const Context &c, int level, int n_rot,
                          const int *rots, uint32_t *const *out_b,
                          uint32_t *const *out_a, const uint32_t *b,
                          const uint32_t *a, const uint32_t *const *key_b,
                          const uint32_t *const *key_a, bool moddown,
-                         uint32_t *ws, Sink &s);
+                         int chunk_rot, uint32_t *ws, Sink &s);
 
 }  // namespace ck

@@ -1,0 +1,161 @@
+// Batched plans: many ciphertexts (at any mix of levels) through one
+// operation, each step merged across them into a single launch.
+//
+// The reference runs HMult / hoisted rotations one ciphertext at a time
+// (src/core/Context.cpp:862-871, src/extension/Hoist.cu:578-691). Here a
+// plan is built once: the same op code that drives the single-ciphertext
+// path (ck_ops.cpp) is replayed per ciphertext into a PlanSink, which
+// appends each step's per-limb jobs to that step's table; the tables and a
+// workspace for every ciphertext are uploaded at creation. Running a plan
+// is then one launch per step (table-driven kernels), with no allocation,
+// copy or synchronisation, so it can be captured into a HIP graph.
+#include "ck_plan.h"
+
+#include <cstring>
+
+namespace ck {
+
+namespace {
+
+constexpr size_t kLimb = 65536;
+
+template <typename T>
+int upload_vec(Plan &p, T **dst, const std::vector<T> &v) {
+  *dst = nullptr;
+  if (v.empty()) return kOk;
+  int rc = check_hip(hipMalloc((void **)dst, v.size() * sizeof(T)),
+                     "hipMalloc(plan table)");
+  if (rc) return rc;
+  p.allocations.push_back(*dst);
+  return check_hip(hipMemcpy(*dst, v.data(), v.size() * sizeof(T),
+                             hipMemcpyHostToDevice),
+                   "hipMemcpy(plan table)");
+}
+
+}  // namespace
+
+PlanStage &PlanSink::stage(PlanStage::Kind kind) {
+  if (cursor_ == plan_.stages.size()) {
+    plan_.stages.emplace_back();
+    plan_.stages.back().kind = kind;
+  }
+  PlanStage &s = plan_.stages[cursor_++];
+  if (s.kind != kind) mismatch_ = true;
+  return s;
+}
+
+int PlanSink::pass(bool forward, int flags, const std::vector<PassJob> &jobs) {
+  PlanStage &s = stage(forward ? PlanStage::kFwd : PlanStage::kInv);
+  s.flags_any |= flags;
+  for (const PassJob &j : jobs) {
+    LimbJob J;
+    std::memset(&J, 0, sizeof(J));
+    J.src = j.src + (size_t)j.src_limb * kLimb;
+    J.dst = j.dst + (size_t)j.dst_limb * kLimb;
+    J.src2 = (j.src2 && j.src2_limb >= 0) ? j.src2 + (size_t)j.src2_limb * kLimb
+                                          : nullptr;
+    J.bsrc = j.bsrc;
+    J.btab = j.btab ? j.btab + j.btab_off : nullptr;
+    J.bs = (uint16_t)j.bs;
+    J.c1 = j.c1;
+    J.c2 = j.c2;
+    J.c3 = j.c3;
+    J.prime = (uint16_t)j.prime;
+    J.flags = flags;
+    s.jobs.push_back(J);
+  }
+  return kOk;
+}
+
+int PlanSink::copy(uint32_t *, const uint32_t *, size_t) {
+  set_error("device copies are not part of batched plans");
+  return kErrUnsupported;
+}
+
+int PlanSink::key_accum(const EwNp &np, const KeyAccumArgs &A) {
+  PlanStage &s = stage(PlanStage::kKeyAccum);
+  const uint16_t item = (uint16_t)s.ka.size();
+  s.ka.push_back(KeyAccumItem{np, A});
+  for (int y = 0; y < np.num_total; y++) s.ew.push_back(EwJob{item, (uint16_t)y});
+  return kOk;
+}
+
+int PlanSink::bs_fused(const EwNp &np, const BsFusedArgs &A) {
+  PlanStage &s = stage(PlanStage::kBsFused);
+  const uint16_t item = (uint16_t)s.bs.size();
+  s.bs.push_back(BsFusedItem{np, A});
+  for (int y = 0; y < np.num_total; y++) s.ew.push_back(EwJob{item, (uint16_t)y});
+  return kOk;
+}
+
+int PlanSink::permute(const EwNp &, const EwOperand &, const EwOperand &,
+                      uint32_t, int) {
+  set_error("permutations are not part of batched plans");
+  return kErrUnsupported;
+}
+
+int plan_finish(Plan &p, bool mismatch) {
+  if (mismatch) {
+    set_error("plan items do not share one step sequence");
+    return kErrUnsupported;
+  }
+  for (PlanStage &s : p.stages) {
+    if (s.ka.size() > 65535 || s.bs.size() > 65535) {
+      set_error("too many items in one plan step");
+      return kErrUnsupported;
+    }
+    int rc = upload_vec(p, &s.d_jobs, s.jobs);
+    if (!rc) rc = upload_vec(p, &s.d_ka, s.ka);
+    if (!rc) rc = upload_vec(p, &s.d_bs, s.bs);
+    if (!rc) rc = upload_vec(p, &s.d_ew, s.ew);
+    if (rc) return rc;
+  }
+  return kOk;
+}
+
+int plan_alloc_workspace(Plan &p, size_t limbs, uint32_t **ws) {
+  *ws = nullptr;
+  if (limbs == 0) return kOk;
+  int rc = check_hip(hipMalloc((void **)ws, limbs * kLimb * sizeof(uint32_t)),
+                     "hipMalloc(plan workspace)");
+  if (rc) return rc;
+  p.allocations.push_back(*ws);
+  p.workspace_bytes += limbs * kLimb * sizeof(uint32_t);
+  return kOk;
+}
+
+int plan_run(const Plan &p, hipStream_t st) {
+  const Context &c = *p.ctx;
+  for (const PlanStage &s : p.stages) {
+    switch (s.kind) {
+      case PlanStage::kFwd:
+        launch_ntt_fwd_table(c, s.d_jobs, (int)s.jobs.size(), s.flags_any, st);
+        break;
+      case PlanStage::kInv:
+        launch_intt_table(c, s.d_jobs, (int)s.jobs.size(), s.flags_any, st);
+        break;
+      case PlanStage::kKeyAccum:
+        launch_key_accum_table(c, s.d_ka, s.d_ew, (int)s.ew.size(), st);
+        break;
+      case PlanStage::kBsFused:
+        launch_bs_fused_table(c, s.d_bs, s.d_ew, (int)s.ew.size(), st);
+        break;
+    }
+  }
+  return check_hip(hipGetLastError(), "plan run");
+}
+
+void plan_destroy(Plan *p) {
+  if (!p) return;
+  for (void *a : p->allocations) (void)hipFree(a);
+  delete p;
+}
+
+int plan_launches(const Plan &p) {
+  int n = 0;
+  for (const PlanStage &s : p.stages)
+    n += (s.kind == PlanStage::kFwd || s.kind == PlanStage::kInv) ? 2 : 1;
+  return n;
+}
+
+}  // namespace ck

@@ -1,0 +1,58 @@
+// Batched plans (see ck_plan.cpp).
+#pragma once
+
+#include "ck_ops.h"
+
+namespace ck {
+
+// One step of a plan: a transform pass over every job's limb, or one
+// key-accumulation kernel over every item's limbs.
+struct PlanStage {
+  enum Kind { kFwd, kInv, kKeyAccum, kBsFused } kind = kFwd;
+  int flags_any = 0;
+  std::vector<LimbJob> jobs;
+  std::vector<KeyAccumItem> ka;
+  std::vector<BsFusedItem> bs;
+  std::vector<EwJob> ew;
+  LimbJob *d_jobs = nullptr;
+  KeyAccumItem *d_ka = nullptr;
+  BsFusedItem *d_bs = nullptr;
+  EwJob *d_ew = nullptr;
+};
+
+struct Plan {
+  const Context *ctx = nullptr;
+  std::vector<PlanStage> stages;
+  std::vector<void *> allocations;
+  size_t workspace_bytes = 0;
+  int items = 0;
+};
+
+// Records the ops of one item after another; begin_item() rewinds to the
+// first step so that step k of every item lands in the same table.
+class PlanSink : public Sink {
+ public:
+  explicit PlanSink(Plan &p) : plan_(p) {}
+  void begin_item() { cursor_ = 0; }
+  bool mismatch() const { return mismatch_; }
+  int pass(bool forward, int flags, const std::vector<PassJob> &jobs) override;
+  int copy(uint32_t *dst, const uint32_t *src, size_t words) override;
+  int key_accum(const EwNp &np, const KeyAccumArgs &A) override;
+  int bs_fused(const EwNp &np, const BsFusedArgs &A) override;
+  int permute(const EwNp &np, const EwOperand &x, const EwOperand &out,
+              uint32_t g, int npoly) override;
+
+ private:
+  PlanStage &stage(PlanStage::Kind kind);
+  Plan &plan_;
+  size_t cursor_ = 0;
+  bool mismatch_ = false;
+};
+
+int plan_alloc_workspace(Plan &p, size_t limbs, uint32_t **ws);
+int plan_finish(Plan &p, bool mismatch);
+int plan_run(const Plan &p, hipStream_t st);
+void plan_destroy(Plan *p);
+int plan_launches(const Plan &p);
+
+}  // namespace ck

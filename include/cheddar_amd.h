@@ -220,6 +220,37 @@ int ck_hrot_hoisted(ck_context *ctx, int level, int n_rot, const int *rots,
                     const uint32_t *const *key_b, const uint32_t *const *key_a,
                     int moddown, void *stream);
 
+/* ---- Batched plans -------------------------------------------------------- */
+/* A plan runs one operation over many ciphertexts (any mix of levels) with
+ * each step merged into a single launch across them. It has no reference
+ * counterpart as an object: it replaces loops of Context::HMult
+ * (src/core/Context.cpp:862-871) and of HoistHandler::EvaluateBabyStep
+ * (src/extension/Hoist.cu:578-691) over a batch. Creation allocates the
+ * plan's workspace and uploads its launch tables; ck_plan_run only launches
+ * kernels (no allocation, copy or synchronisation), so it may be captured
+ * into a HIP graph. The plan keeps the pointers it was given: buffers must
+ * stay allocated until ck_plan_destroy. */
+typedef struct ck_plan ck_plan;
+/* out_* [i] = HMult(ct1[i], ct2[i]) at levels[i] with one shared key. */
+int ck_plan_hmult(ck_context *ctx, int n, const int *levels, int rescale,
+                  uint32_t *const *out_b, uint32_t *const *out_a,
+                  const uint32_t *const *b1, const uint32_t *const *a1,
+                  const uint32_t *const *b2, const uint32_t *const *a2,
+                  const uint32_t *const *key_b, const uint32_t *const *key_a,
+                  ck_plan **out);
+/* out_*[i * n_rot + r] = hoisted rotation r of ciphertext i (as
+ * ck_hrot_hoisted), one shared set of n_rot rotation keys. */
+int ck_plan_hrot_hoisted(ck_context *ctx, int n, const int *levels,
+                         const uint32_t *const *b, const uint32_t *const *a,
+                         int n_rot, const int *rots,
+                         const uint32_t *const *key_b,
+                         const uint32_t *const *key_a, uint32_t *const *out_b,
+                         uint32_t *const *out_a, int moddown, ck_plan **out);
+int ck_plan_run(ck_plan *plan, void *stream);
+/* out[4]: items, steps, kernel launches per run, workspace bytes */
+int ck_plan_info(const ck_plan *plan, long long *out);
+int ck_plan_destroy(ck_plan *plan);
+
 /* ---- Per-kernel timing ---------------------------------------------------- */
 /* Replaces the wall-clock profiling macros of the reference's test harness
  * (unittest/Testbed.h:24-47) with device-side timing: while enabled, every

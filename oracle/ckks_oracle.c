@@ -1023,24 +1023,16 @@ int or_hmult(const or_param *p, int level, const uint32_t *b1,
   return rc;
 }
 
-/* Context::HRot (Context.cpp:776-791): MultKey on ax with accum.bx += P*bx
- * (:685-699), ModDown both, then Permute by 5^rot (Context.cpp:502-527). */
-int or_hrot(const or_param *p, int level, const uint32_t *b,
-            const uint32_t *a, const uint32_t *rk_b, const uint32_t *rk_a,
-            int rot, uint32_t *out_b, uint32_t *out_a) {
+/* MultKey on ax with accum.bx += P*bx (Context.cpp:685-699), ModDown both,
+ * then Permute by the Galois factor g (Context.cpp:502-527). */
+static int hrot_galois(const or_param *p, int level, const uint32_t *b,
+                       const uint32_t *a, const uint32_t *rk_b,
+                       const uint32_t *rk_a, uint32_t g, uint32_t *out_b,
+                       uint32_t *out_a) {
   const int n = p->n, alpha = p->alpha;
-  if (level < 0 || level > p->max_level) return -1;
   int nm = p->lv_main[level], nt = p->lv_ter[level];
   int num_q = nm + nt, tot = num_q + alpha;
   size_t qs = (size_t)num_q * n;
-  int slots = n / 2;
-  rot %= slots;
-  if (rot < 0) rot += slots;
-  if (rot == 0) {
-    memcpy(out_b, b, sizeof(uint32_t) * qs);
-    memcpy(out_a, a, sizeof(uint32_t) * qs);
-    return 0;
-  }
   uint32_t *acc = (uint32_t *)malloc(sizeof(uint32_t) * 2 * (size_t)tot * n);
   uint32_t *tmp = (uint32_t *)malloc(sizeof(uint32_t) * 2 * qs);
   int rc = or_mult_key_no_moddown(p, level, a, b, NULL, rk_b, rk_a, acc,
@@ -1049,13 +1041,40 @@ int or_hrot(const or_param *p, int level, const uint32_t *b,
   if (rc == 0)
     rc = or_moddown(p, level, OR_MODDOWN, acc + (size_t)tot * n, tmp + qs);
   if (rc == 0) {
-    uint32_t g = or_galois_factor(p, rot);
     or_permute(p, num_q, g, tmp, out_b);
     or_permute(p, num_q, g, tmp + qs, out_a);
   }
   free(acc);
   free(tmp);
   return rc;
+}
+
+/* Context::HRot (Context.cpp:776-791): rotation by rot (reduced mod N/2;
+ * 0 copies), Galois factor 5^rot. */
+int or_hrot(const or_param *p, int level, const uint32_t *b,
+            const uint32_t *a, const uint32_t *rk_b, const uint32_t *rk_a,
+            int rot, uint32_t *out_b, uint32_t *out_a) {
+  if (level < 0 || level > p->max_level) return -1;
+  const int slots = p->n / 2;
+  size_t qs = (size_t)(p->lv_main[level] + p->lv_ter[level]) * p->n;
+  rot %= slots;
+  if (rot < 0) rot += slots;
+  if (rot == 0) {
+    memcpy(out_b, b, sizeof(uint32_t) * qs);
+    memcpy(out_a, a, sizeof(uint32_t) * qs);
+    return 0;
+  }
+  return hrot_galois(p, level, b, a, rk_b, rk_a, or_galois_factor(p, rot),
+                     out_b, out_a);
+}
+
+/* Context::HConj (Context.cpp:793-798): MultKey then Permute by 2N - 1 */
+int or_hconj(const or_param *p, int level, const uint32_t *b,
+             const uint32_t *a, const uint32_t *ck_b, const uint32_t *ck_a,
+             uint32_t *out_b, uint32_t *out_a) {
+  if (level < 0 || level > p->max_level) return -1;
+  return hrot_galois(p, level, b, a, ck_b, ck_a, or_galois_factor(p, -1),
+                     out_b, out_a);
 }
 
 /* HoistHandler baby step for one rotation: MultKeyNoModDown(a) over the

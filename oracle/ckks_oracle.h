@@ -186,6 +186,10 @@ int or_hrot(const or_param *p, int level, const uint32_t *b,
             const uint32_t *a, const uint32_t *rk_b, const uint32_t *rk_a,
             int rot, uint32_t *out_b, uint32_t *out_a);
 
+/* HConj = MultKey + Permute(2N - 1) (Context.cpp:793-798) */
+int or_hconj(const or_param *p, int level, const uint32_t *b,
+             const uint32_t *a, const uint32_t *ck_b, const uint32_t *ck_a,
+             uint32_t *out_b, uint32_t *out_a);
 /* Hoisted baby step for one rotation (HoistHandler::EvaluateBabyStep,
  * src/extension/Hoist.cu:578-691, BSFusedKernel :21-78): out over
  * (num_q + alpha) limbs, rot in [1, N/2). */

@@ -82,6 +82,7 @@ def _load():
         "or_mult_key_no_moddown": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [_u32p] * 7),
         "or_hmult": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [_u32p] * 6 + [ctypes.c_int, _u32p, _u32p]),
         "or_hrot": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [_u32p] * 4 + [ctypes.c_int, _u32p, _u32p]),
+        "or_hconj": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [_u32p] * 6),
         "or_hoisted_baby_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [_u32p] * 4 +
                                  [ctypes.c_int, _u32p, _u32p]),
     }
@@ -326,6 +327,16 @@ class Param:
         rc = lib.or_hmult(self._h, level, *[_p(a) for a in ins], int(rescale), _p(ob), _p(oa))
         if rc != 0:
             raise ValueError("or_hmult failed")
+        return ob, oa
+
+    def hconj(self, level, ct, ck):
+        nq = self.num_q(level)
+        ob = np.empty((nq, self.n), dtype=np.uint32)
+        oa = np.empty((nq, self.n), dtype=np.uint32)
+        ins = [np.ascontiguousarray(v, dtype=np.uint32) for v in (ct[0], ct[1], ck[0], ck[1])]
+        rc = lib.or_hconj(self._h, level, *[_p(a) for a in ins], _p(ob), _p(oa))
+        if rc != 0:
+            raise ValueError("or_hconj failed")
         return ob, oa
 
     def hoisted_baby_step(self, level, ct, rk, rot):

@@ -1,0 +1,32 @@
+"""The C++ host facade (cheddar-fhe_amd/host/include/cheddar/, the
+reference's Parameter / Context / Ciphertext / EvaluationKey API) driven by
+tests/cpp/test_context.cpp: parameter JSON in the reference's schema,
+HMult / Relinearize(Rescale) / Rescale / HRot / HConj / hoisted HRot /
+MultKeyNoModDown / Add / Sub / Neg on the GPU, each output compared word
+for word with the C oracle inside that program."""
+import os
+import subprocess
+
+import pytest
+
+from conftest import GOLDEN, PKG, REPO
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(REPO, "tests", "cpp", "test_context")
+
+
+@pytest.mark.parametrize("params,level", [
+    (os.path.join(PKG, "params", "rns24_aux8.json"), 23),
+    (os.path.join(PKG, "params", "rns24_aux8.json"), 6),
+    (os.path.join(GOLDEN, "params", "bootparam_40.json"), 20),
+])
+def test_cpp_facade(params, level):
+    assert os.path.exists(BIN), "tests/cpp/test_context is not built (run build())"
+    print(f"facade test {os.path.basename(params)} level {level}", flush=True)
+    r = subprocess.run([BIN, params, str(level)], capture_output=True, text=True,
+                       timeout=240)
+    print(r.stdout[-4000:])
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "OK: 0 failure(s)" in r.stdout
+    assert "FAIL" not in r.stdout

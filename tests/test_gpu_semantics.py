@@ -66,6 +66,9 @@ def test_gpu_hrot_hconj_decrypt(ck, ctx24, oracle_rns24, keys, rot):
     out = [torch.empty_like(d[0]) for _ in range(2)]
     if rot == -1:
         ctx24.hconj(out, d, dev_key(*key), level)
+        want = P.hconj(level, ct, key)
+        assert_same(ck.to_host(out[0]), want[0], "hconj b")
+        assert_same(ck.to_host(out[1]), want[1], "hconj a")
     else:
         ctx24.hrot(out, d, dev_key(*key), rot, level)
         want = P.hrot(level, ct, key, rot)

@@ -109,6 +109,8 @@ def test_hconj_decrypts_to_conjugate(oracle_rns24, keys24):
     ra = P.moddown(acc_a, level, O.OR_MODDOWN)
     g = P.galois(-1)
     out = (P.permute(rb, g), P.permute(ra, g))
+    got = P.hconj(level, (b, a), (kb, ka))
+    assert np.array_equal(got[0], out[0]) and np.array_equal(got[1], out[1])
     want = automorph_coef(m, g)
     ks = [0, 1, 2, P.n - 1, 4321]
     for k, v in zip(ks, K.decrypt(out, level, ks)):
