@@ -102,13 +102,21 @@ def _short(sym):
     return s[:-7] if s.endswith("_kernel") else s
 
 
+_KERNELS = ("ntt_fwd_cols", "intt_cols", "ntt_fwd_rows", "intt_rows", "key_accum",
+            "key_accum_table", "bs_fused", "bs_fused_table", "tensor", "permute")
+
+
 def _limbs(short, grid_x, grid_y, wg_x):
+    """limbs of one launch from its grid (None for kernels not of this
+    library)"""
+    if short not in _KERNELS:
+        return None
     wgs = grid_x // max(1, wg_x)
     if short in ("ntt_fwd_cols", "intt_cols"):
-        return wgs // 8
-    if grid_y > 1:
-        return grid_y
-    return None
+        return wgs // 16
+    if short in ("bs_fused", "bs_fused_table"):
+        return wgs // 64
+    return grid_y
 
 
 def _read_csv(path):
@@ -155,7 +163,7 @@ def pmc_traffic(fetch_csv, write_csv):
             wg = int(r["Workgroup_Size"])
             grid = int(r["Grid_Size"])
             if short in ("ntt_fwd_cols", "intt_cols"):
-                n = grid // wg // 8
+                n = grid // wg // 16
             elif short in ("ntt_fwd_rows", "intt_rows"):
                 n = grid // wg // 16
             elif short in ("key_accum", "tensor", "permute"):

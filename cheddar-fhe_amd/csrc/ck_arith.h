@@ -89,9 +89,11 @@ struct ModF {
   double qinv;  // 1/q rounded
 };
 
-// a * w mod q for integer-valued doubles; |a| < 2^48, 0 <= w < q < 2^31,
-// wq = w/q (any correctly rounded value). Result r == a*w (mod q) exactly,
-// |r| < 1.5 q + 2^26.
+// a * w mod q for integer-valued doubles; |a| < 2^50, 0 <= w < q < 2^31,
+// wq = w/q (correctly rounded). Result r == a*w (mod q) exactly with
+// |r| < q/2 + |a w| 2^-52 < 0.75 q: k is within 1/2 + |a w / q| 2^-52 of
+// a w / q, hi - k q is an integer below 2^30 in magnitude (exact in the
+// fma) and lo is the exact rounding error of hi.
 __device__ __forceinline__ double mulmod_f(double a, double w, double wq,
                                            double q) {
   double hi = a * w;
@@ -100,18 +102,21 @@ __device__ __forceinline__ double mulmod_f(double a, double w, double wq,
   return __builtin_fma(-k, q, hi) + lo;
 }
 
-// |x| < 2^50 -> x mod q in (-q/2 - 1, q/2 + 1) (one rounding step)
+// |x| < 2^50 -> x mod q in (-0.75 q, 0.75 q) (one rounding step; within
+// (-q/2 - 1, q/2 + 1) for |x| < 2^40)
 __device__ __forceinline__ double reduce_f(double x, const ModF &m) {
   double k = __builtin_rint(x * m.qinv);
   return __builtin_fma(-k, m.q, x);
 }
 
+// |r| < q (a mulmod_f or reduce_f result) -> canonical [0, q) as uint32
+__device__ __forceinline__ uint32_t canon_small(double r, double q) {
+  return (uint32_t)(r < 0.0 ? r + q : r);
+}
+
 // |x| < 2^50 -> canonical [0, q) as uint32
 __device__ __forceinline__ uint32_t canon_f(double x, const ModF &m) {
-  double r = reduce_f(x, m);
-  r = r < 0.0 ? r + m.q : r;
-  r = r >= m.q ? r - m.q : r;
-  return (uint32_t)r;
+  return canon_small(reduce_f(x, m), m.q);
 }
 
 // |x| < 2^50 -> centred [-(q-1)/2, (q-1)/2] as int32

@@ -13,9 +13,11 @@
 //   * row kernels (forward stages 8..15, inverse stages 0..7): a 256-thread
 //     workgroup owns 16 consecutive rows = one contiguous 16 KiB block,
 //     moved with 16-byte loads/stores that cover whole cache lines.
-//   * column kernels (forward stages 0..7, inverse 8..15): a 512-thread
-//     workgroup owns all 256 rows x 32 columns (128 B per row segment).
-// Each thread runs two radix-16 register phases on 16 values. Global data
+//   * column kernels (forward stages 0..7, inverse 8..15): a 256-thread
+//     workgroup owns all 256 rows x 16 columns; every load and store is a
+//     64-byte row segment per 16 lanes, read and written straight from
+//     registers (32 KiB of LDS: four workgroups and more fit a CU).
+// Each thread runs two radix-16 register phases on 16 values. Row tiles
 // move through an int32 LDS staging tile with 16-byte accesses; the hand-off
 // between the two phases keeps the lazy doubles in an LDS tile aliased with
 // the staging tile (row tiles use an XOR swizzle so both phase layouts are
@@ -23,7 +25,7 @@
 // and issued before the data loads. Butterflies use the exact FP64 modular
 // product of ck_arith.h with lazy ranges (integer doubles below 2^40); the
 // intermediate between the two passes of a transform is stored as a lazy
-// int32 in (-q/2 - 2, q/2 + 2) and only final outputs are made canonical.
+// int32 in (-0.75 q, 0.75 q) and only final outputs are made canonical.
 // Each thread writes a phase's results back to exactly the LDS words it read
 // for that phase, so only the hand-offs between threads need a barrier.
 #include "ck_internal.h"
@@ -35,8 +37,8 @@ constexpr int kN = 65536;
 constexpr int kRowThreads = 256;
 constexpr int kRowTile = 16;                  // rows per row-kernel workgroup
 constexpr int kRowTiles = 256 / kRowTile;     // per limb
-constexpr int kColThreads = 512;
-constexpr int kColTile = 32;                  // columns per column workgroup
+constexpr int kColThreads = 256;
+constexpr int kColTile = 16;                  // columns per column workgroup
 constexpr int kColTiles = 256 / kColTile;     // per limb
 
 // forward radix-16 (4 CT stages), element index bits high to low; twiddle
@@ -120,7 +122,25 @@ struct JobByValue {
 
 struct JobTable {
   const LimbJob *__restrict__ jobs;
-  __device__ __forceinline__ LimbJob job(int y) const { return jobs[y]; }
+  // the record is read-only and uniform per workgroup: scalar loads
+  // through the constant address space
+  __device__ __forceinline__ LimbJob job(int y) const {
+    typedef const __attribute__((address_space(4))) LimbJob CJob;
+    CJob *p = (CJob *)jobs + y;
+    LimbJob J;
+    J.src = p->src;
+    J.dst = p->dst;
+    J.src2 = p->src2;
+    J.bsrc = p->bsrc;
+    J.btab = p->btab;
+    J.c1 = p->c1;
+    J.c2 = p->c2;
+    J.c3 = p->c3;
+    J.prime = p->prime;
+    J.bs = p->bs;
+    J.flags = p->flags;
+    return J;
+  }
 };
 
 // 16 KiB contiguous row tile <-> LDS, 16-byte accesses
@@ -151,17 +171,15 @@ __device__ __forceinline__ void lds_to_tile_rows(const int32_t *lds, uint32_t *g
   }
 }
 
-// column tile (256 rows x 32 columns starting at column c0) <-> LDS [256][32]
-__device__ __forceinline__ void tile_cols_to_lds(const uint32_t *g, int c0,
-                                                 int32_t *lds) {
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int idx = 512 * i + threadIdx.x;  // 16-byte chunk index
-    const int r = idx >> 3, c4 = (idx & 7) * 4;
-    const uint4 v = *reinterpret_cast<const uint4 *>(g + r * 256 + c0 + c4);
-    *reinterpret_cast<int4 *>(lds + r * 32 + c4) =
-        make_int4((int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w);
-  }
+// Column tile of a launch: 16 tiles of 16 columns per limb; workgroup i
+// runs on XCD i mod 8, and tiles 2x and 2x + 1 (which share 128-byte lines)
+// are placed on the same XCD so a line is fetched into one L2 only.
+struct ColTile {
+  int y, c0;
+};
+__device__ __forceinline__ ColTile col_tile() {
+  const int t = ((blockIdx.x & 7) << 1) | ((blockIdx.x >> 3) & 1);
+  return ColTile{(int)(blockIdx.x >> 4), t * kColTile};
 }
 
 // BaseConv computed straight into the column tile (ModSwitchMatrixMult,
@@ -169,21 +187,25 @@ __device__ __forceinline__ void tile_cols_to_lds(const uint32_t *g, int c0,
 // y = sum_j v_j * T_j mod t over bs centred int32 source limbs, with
 // T_j = hi_j * 2^16 + lo_j so every partial sum stays an exact double
 // (|v| < 2^30: |sum v hi| < 2^49, |sum v lo| < 2^50 for bs <= 16).
+// Output: int32 tile [256][16] in LDS, 16-byte source loads.
 __device__ __forceinline__ void bconv_cols_to_lds(const int32_t *bsrc, int bs,
                                                   const double *tab, int c0,
                                                   const ModF &m,
                                                   int32_t *lds) {
+  // the table row is read-only and uniform: scalar loads (constant space)
+  typedef const __attribute__((address_space(4))) double cdouble;
+  cdouble *ctab = (cdouble *)tab;
   double th[kMaxBconvSrc], tl[kMaxBconvSrc];
 #pragma unroll
   for (int j = 0; j < kMaxBconvSrc; j++) {
     if (j < bs) {
-      th[j] = tab[2 * j];
-      tl[j] = tab[2 * j + 1];
+      th[j] = ctab[2 * j];
+      tl[j] = ctab[2 * j + 1];
     }
   }
   for (int i = 0; i < 4; i++) {
-    const int idx = 512 * i + threadIdx.x;
-    const int r = idx >> 3, c4 = (idx & 7) * 4;
+    const int idx = kColThreads * i + threadIdx.x;  // 16-byte chunk index
+    const int r = idx >> 2, c4 = (idx & 3) * 4;
     const int32_t *p = bsrc + r * 256 + c0 + c4;
     double sh[4] = {0.0, 0.0, 0.0, 0.0}, sl[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -207,24 +229,12 @@ __device__ __forceinline__ void bconv_cols_to_lds(const int32_t *bsrc, int bs,
       const double k = __builtin_rint(big * m.qinv);
       op[e] = (int32_t)reduce_f(__builtin_fma(-k, m.q, big) + sl[e], m);
     }
-    *reinterpret_cast<int4 *>(lds + r * 32 + c4) = o;
-  }
-}
-
-__device__ __forceinline__ void lds_to_tile_cols(const int32_t *lds, uint32_t *g,
-                                                 int c0) {
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int idx = 512 * i + threadIdx.x;
-    const int r = idx >> 3, c4 = (idx & 7) * 4;
-    const int4 v = *reinterpret_cast<const int4 *>(lds + r * 32 + c4);
-    *reinterpret_cast<uint4 *>(g + r * 256 + c0 + c4) =
-        make_uint4((uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z, (uint32_t)v.w);
+    *reinterpret_cast<int4 *>(lds + r * kColTile + c4) = o;
   }
 }
 
 // Final value of an element: canonical u32, or lazy int32 in
-// (-q/2 - 2, q/2 + 2) for a pass-1 intermediate that pass 2 reads back.
+// (-0.75 q, 0.75 q) for a pass-1 intermediate that pass 2 reads back.
 __device__ __forceinline__ int32_t lazy_i32(double x, const ModF &m) {
   return (int32_t)reduce_f(x, m);
 }
@@ -233,22 +243,19 @@ __device__ __forceinline__ int32_t lazy_i32(double x, const ModF &m) {
 // Forward, column stages s = 0..7 -> lazy int32 intermediate
 // ---------------------------------------------------------------------------
 template <class D>
-__global__ __launch_bounds__(kColThreads) void ntt_fwd_cols(
+__global__ __launch_bounds__(kColThreads, 5) void ntt_fwd_cols(
     const D desc, const PrimeTab P, const uint32_t *__restrict__ tw) {
-  // int32 staging tile [256][32] aliased with the double exchange tile
+  // int32 BaseConv tile [256][16] aliased with the double exchange tile
   __shared__ double ldsd[256 * kColTile];
   int32_t *lds = reinterpret_cast<int32_t *>(ldsd);
-  // XCD-aware: workgroups are dealt round-robin to the 8 XCDs, so column
-  // tile (blockIdx.x & 7) of every limb runs on the same XCD and the source
-  // tiles a BaseConv input shares stay in that XCD's L2.
-  const int y = blockIdx.x >> 3;
-  const int c0 = (blockIdx.x & 7) * kColTile;
-  const LimbJob J = desc.job(y);
+  const ColTile ct = col_tile();
+  const int c0 = ct.c0;
+  const LimbJob J = desc.job(ct.y);
   const double q = P.qd[J.prime], qr = P.qr[J.prime];
   const ModF m{q, qr};
   const uint32_t *__restrict__ w = tw + (size_t)J.prime * kN;
-  const int cc = threadIdx.x & 31;
-  const int rb = threadIdx.x >> 5;  // 0..15
+  const int cc = threadIdx.x & 15;
+  const int rb = threadIdx.x >> 4;  // 0..15
 
   // phase 1 twiddles psi_rev[2^s + blk] (uniform), phase 2 psi_rev[2^(4+s)
   // + rb 2^s + blk]
@@ -260,40 +267,43 @@ __global__ __launch_bounds__(kColThreads) void ntt_fwd_cols(
       t1[(1 << s) - 1 + blk] = w[(1 << s) + blk];
       t2[(1 << s) - 1 + blk] = w[(16 << s) + (rb << s) + blk];
     }
-  if (J.flags & kNttBconvInput)
-    bconv_cols_to_lds(J.bsrc, J.bs, J.btab, c0, m, lds);
-  else
-    tile_cols_to_lds(J.src, c0, lds);
-  __syncthreads();
-
+  // phase 1 holds rows rb + 16 k of column c0 + cc
   double x[16];
+  if (J.flags & kNttBconvInput) {
+    bconv_cols_to_lds(J.bsrc, J.bs, J.btab, c0, m, lds);
+    __syncthreads();
 #pragma unroll
-  for (int k = 0; k < 16; k++) x[k] = (double)lds[(rb + 16 * k) * 32 + cc];
+    for (int k = 0; k < 16; k++) x[k] = (double)lds[(rb + 16 * k) * kColTile + cc];
+    __syncthreads();  // the staging words are reused by the double tile
+  } else {
+    const uint32_t *__restrict__ g = J.src + c0 + cc;
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = (double)g[(rb + 16 * k) * 256];
+  }
   if (J.flags & kNttPreMul) {
     const double cv = (double)J.c1, cq = cv * qr;
 #pragma unroll
     for (int k = 0; k < 16; k++) x[k] = mulmod_f(x[k], cv, cq, q);
   }
   ct16(x, t1, q, qr);
-  __syncthreads();  // staging words are reused by the double tile
 #pragma unroll
-  for (int k = 0; k < 16; k++) ldsd[(rb + 16 * k) * 32 + cc] = x[k];
+  for (int k = 0; k < 16; k++) ldsd[(rb + 16 * k) * kColTile + cc] = x[k];
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < 16; j++) x[j] = ldsd[(rb * 16 + j) * 32 + cc];
+  for (int j = 0; j < 16; j++) x[j] = ldsd[(rb * 16 + j) * kColTile + cc];
   ct16(x, t2, q, qr);
-  __syncthreads();
+  // rows rb * 16 + j, lazy int32 for the row pass
+  uint32_t *__restrict__ o = J.dst + c0 + cc;
 #pragma unroll
-  for (int j = 0; j < 16; j++) lds[(rb * 16 + j) * 32 + cc] = lazy_i32(x[j], m);
-  __syncthreads();
-  lds_to_tile_cols(lds, J.dst, c0);
+  for (int j = 0; j < 16; j++)
+    o[(rb * 16 + j) * 256] = (uint32_t)lazy_i32(x[j], m);
 }
 
 // ---------------------------------------------------------------------------
 // Forward, row stages s = 8..15 (+ optional ModDown epilogue)
 // ---------------------------------------------------------------------------
 template <class D>
-__global__ __launch_bounds__(kRowThreads) void ntt_fwd_rows(
+__global__ __launch_bounds__(kRowThreads, 5) void ntt_fwd_rows(
     const D desc, const PrimeTab P, const uint32_t *__restrict__ tw) {
   __shared__ double ldsd[kRowTile * 256];
   int32_t *lds = reinterpret_cast<int32_t *>(ldsd);
@@ -361,7 +371,7 @@ __global__ __launch_bounds__(kRowThreads) void ntt_fwd_rows(
     }
 #pragma unroll
     for (int j = 0; j < 16; j++)
-      out[j] = canon_f(mulmod_f(a2[j] - x[j], ip, ipq, q), m);
+      out[j] = canon_small(mulmod_f(a2[j] - x[j], ip, ipq, q), q);
   } else {
 #pragma unroll
     for (int j = 0; j < 16; j++) out[j] = canon_f(x[j], m);
@@ -377,7 +387,7 @@ __global__ __launch_bounds__(kRowThreads) void ntt_fwd_rows(
 // Inverse, row stages u = 0..7 -> lazy int32 intermediate
 // ---------------------------------------------------------------------------
 template <class D>
-__global__ __launch_bounds__(kRowThreads) void intt_rows(
+__global__ __launch_bounds__(kRowThreads, 5) void intt_rows(
     const D desc, const PrimeTab P, const uint32_t *__restrict__ tw) {
   __shared__ double ldsd[kRowTile * 256];
   int32_t *lds = reinterpret_cast<int32_t *>(ldsd);
@@ -433,9 +443,11 @@ __global__ __launch_bounds__(kRowThreads) void intt_rows(
     }
   }
   gs16(x, t1, q, qr);
-  // GS sums double every stage: reduce before the next four stages
+  // GS sums double every stage: from inputs below 2^31 they stay below 2^39
+  // over all eight stages, well inside mulmod_f's range, so the hand-off
+  // needs no reduction
 #pragma unroll
-  for (int j = 0; j < 16; j++) ldsd[rlds(rl, 16 * t16 + j)] = reduce_f(x[j], m);
+  for (int j = 0; j < 16; j++) ldsd[rlds(rl, 16 * t16 + j)] = x[j];
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < 16; k++) x[k] = ldsd[rlds(rl, t16 + 16 * k)];
@@ -451,18 +463,17 @@ __global__ __launch_bounds__(kRowThreads) void intt_rows(
 // Inverse, column stages u = 8..15, then the per-limb constant
 // ---------------------------------------------------------------------------
 template <class D>
-__global__ __launch_bounds__(kColThreads) void intt_cols(
+__global__ __launch_bounds__(kColThreads, 5) void intt_cols(
     const D desc, const PrimeTab P, const uint32_t *__restrict__ tw) {
   __shared__ double ldsd[256 * kColTile];
-  int32_t *lds = reinterpret_cast<int32_t *>(ldsd);
-  const int y = blockIdx.x >> 3;  // same XCD-aware order as ntt_fwd_cols
-  const int c0 = (blockIdx.x & 7) * kColTile;
-  const LimbJob J = desc.job(y);
+  const ColTile ct = col_tile();  // same XCD-aware order as ntt_fwd_cols
+  const int c0 = ct.c0;
+  const LimbJob J = desc.job(ct.y);
   const double q = P.qd[J.prime], qr = P.qr[J.prime];
   const ModF m{q, qr};
   const uint32_t *__restrict__ w = tw + (size_t)J.prime * kN;
-  const int cc = threadIdx.x & 31;
-  const int rb = threadIdx.x >> 5;
+  const int cc = threadIdx.x & 15;
+  const int rb = threadIdx.x >> 4;
 
   // phase 1 psi_inv_rev[2^(7-u) + rb 2^(3-u) + blk],
   // phase 2 psi_inv_rev[2^(3-u) + blk] (uniform)
@@ -474,35 +485,37 @@ __global__ __launch_bounds__(kColThreads) void intt_cols(
       t1[16 - (16 >> u) + blk] = w[(128 >> u) + (rb << (3 - u)) + blk];
       t2[16 - (16 >> u) + blk] = w[(8 >> u) + blk];
     }
-  // the pass-1 intermediate (in dst) is lazy signed int32
-  tile_cols_to_lds(J.dst, c0, lds);
-  __syncthreads();
+  // the pass-1 intermediate (in dst) is lazy signed int32; phase 1 holds
+  // rows rb * 16 + j of column c0 + cc
   double x[16];
+  {
+    const int32_t *__restrict__ g =
+        reinterpret_cast<const int32_t *>(J.dst) + c0 + cc;
 #pragma unroll
-  for (int j = 0; j < 16; j++) x[j] = (double)lds[(rb * 16 + j) * 32 + cc];
+    for (int j = 0; j < 16; j++) x[j] = (double)g[(rb * 16 + j) * 256];
+  }
   gs16(x, t1, q, qr);
-  __syncthreads();  // staging words are reused by the double tile
+  // lazy inputs (|x| < 0.75 q) grow below 2^39 over the eight stages
 #pragma unroll
-  for (int j = 0; j < 16; j++)
-    ldsd[(rb * 16 + j) * 32 + cc] = reduce_f(x[j], m);
+  for (int j = 0; j < 16; j++) ldsd[(rb * 16 + j) * kColTile + cc] = x[j];
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < 16; k++) x[k] = ldsd[(rb + 16 * k) * 32 + cc];
+  for (int k = 0; k < 16; k++) x[k] = ldsd[(rb + 16 * k) * kColTile + cc];
   gs16(x, t2, q, qr);
   const double cv = (double)J.c1, cq = cv * qr;
+  uint32_t *__restrict__ o = J.dst + c0 + cc;
+  // x * c1 lands in (-0.75 q, 0.75 q): canonical [0, q) (above: never,
+  // below 0: + q) or centred [-(q-1)/2, (q-1)/2] (one fix-up either side)
   const bool centred = (J.flags & kInttCentred) != 0;
-  __syncthreads();
+  const double above = centred ? (q - 1.0) * 0.5 : q;
+  const double below = centred ? -(q - 1.0) * 0.5 : 0.0;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
-    const double r0 = reduce_f(mulmod_f(x[k], cv, cq, q), m);
-    // canonical [0, q) or centred [-(q-1)/2, (q-1)/2]
-    const double lo = centred ? -(q - 1.0) * 0.5 : 0.0;
-    double r1 = r0 < lo ? r0 + q : r0;
-    r1 = r1 >= lo + q ? r1 - q : r1;
-    lds[(rb + 16 * k) * 32 + cc] = (int32_t)r1;
+    double r = mulmod_f(x[k], cv, cq, q);
+    r = r > above ? r - q : r;
+    r = r < below ? r + q : r;
+    o[(rb + 16 * k) * 256] = (uint32_t)(int32_t)r;
   }
-  __syncthreads();
-  lds_to_tile_cols(lds, J.dst, c0);
 }
 
 const char *fwd_cols_name(int flags) {

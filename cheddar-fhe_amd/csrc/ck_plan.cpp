@@ -11,6 +11,7 @@
 // copy or synchronisation, so it can be captured into a HIP graph.
 #include "ck_plan.h"
 
+#include <algorithm>
 #include <cstring>
 
 namespace ck {
@@ -94,12 +95,41 @@ int PlanSink::permute(const EwNp &, const EwOperand &, const EwOperand &,
   return kErrUnsupported;
 }
 
+namespace {
+
+int ew_prime(const EwNp &np, int y) {
+  return y < np.num_q ? np.q_prime0 + y : np.aux_prime0 + (y - np.num_q);
+}
+
+// Jobs of one step are independent, so their order is free: grouping them by
+// prime lets consecutive workgroups share the twiddle table (transforms) or
+// the key limb (key products, whose keys every item shares) in cache instead
+// of each item fetching it from HBM.
+// A BaseConv-input pass is the exception: its output limbs of one
+// polynomial re-read the same source tiles, so they stay together.
+void order_by_prime(PlanStage &s) {
+  if (!(s.flags_any & kNttBconvInput))
+    std::stable_sort(s.jobs.begin(), s.jobs.end(),
+                     [](const LimbJob &a, const LimbJob &b) { return a.prime < b.prime; });
+  auto prime_of = [&](const EwJob &j) {
+    return s.kind == PlanStage::kKeyAccum ? ew_prime(s.ka[j.item].np, j.y)
+                                          : ew_prime(s.bs[j.item].np, j.y);
+  };
+  if (s.kind == PlanStage::kKeyAccum || s.kind == PlanStage::kBsFused)
+    std::stable_sort(s.ew.begin(), s.ew.end(), [&](const EwJob &a, const EwJob &b) {
+      return prime_of(a) < prime_of(b);
+    });
+}
+
+}  // namespace
+
 int plan_finish(Plan &p, bool mismatch) {
   if (mismatch) {
     set_error("plan items do not share one step sequence");
     return kErrUnsupported;
   }
   for (PlanStage &s : p.stages) {
+    order_by_prime(s);
     if (s.ka.size() > 65535 || s.bs.size() > 65535) {
       set_error("too many items in one plan step");
       return kErrUnsupported;
