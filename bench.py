@@ -57,7 +57,7 @@ def alg_limbs(ctx, level, name):
         return 2 * n, n
     if kern in ("intt_rows", "intt_cols", "ntt_fwd_rows", "ntt_fwd_cols", "permute"):
         return n, n
-    if kern == "ntt_fwd_cols+bconv":
+    if kern in ("ntt_fwd_cols+bconv", "bconv_mfma"):
         if n == beta * (num_q + alpha) - num_q:   # ModUp: the q limbs, once
             return num_q, n
         polys = n // next_q                        # ModDown(+Rescale)
@@ -240,6 +240,7 @@ def bench_batched(a, torch, ck):
     import random
     prm = ck.load_param_json(PARAMS)
     ctx = ck.Context.from_json(prm)
+    ctx.set_bconv_mfma(None if a.bconv == "auto" else a.bconv == "mfma")
     torch.cuda.set_device(0)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(0x4348454444415202)
@@ -316,6 +317,10 @@ def main():
     ap.add_argument("--rocprof-summary", metavar="KERNEL_TRACE_CSV",
                     help="print a per-kernel summary of a rocprofv3 --kernel-trace "
                          "CSV and exit")
+    ap.add_argument("--bconv", choices=("auto", "fused", "mfma"), default="auto",
+                    help="BaseConv fused into the forward NTT column pass, on the FP64 "
+                         "matrix cores as its own step, or auto (matrix cores in batched "
+                         "plans only)")
     ap.add_argument("--workload", choices=("hmult", "batched"), default="hmult",
                     help="hmult: one HMult+relin+rescale per step at L=24 (default); "
                          "batched: 64 ciphertexts at 8..24 limbs through one HMult "
@@ -343,6 +348,7 @@ def main():
 
     prm = ck.load_param_json(PARAMS)
     ctx = ck.Context.from_json(prm)
+    ctx.set_bconv_mfma(None if a.bconv == "auto" else a.bconv == "mfma")
     torch.cuda.set_device(0)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(0x4348454444415201)
@@ -424,7 +430,7 @@ def main():
         "dtype": "u32",
         "data": "synthetic uniform residues and evaluation key (seeded)",
         "config": {"workload": "HMult+relin+rescale, N=2^16, L=24, alpha=8, dnum=3, "
-                               "plain HIP form", "log_n": 16, "L": ctx.L, "alpha": ctx.alpha,
+                               "plain HIP form", "bconv": a.bconv, "log_n": 16, "L": ctx.L, "alpha": ctx.alpha,
                    "dnum": ctx.dnum, "level": LEVEL, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

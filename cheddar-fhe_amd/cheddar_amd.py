@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libcheddar_amd.so")
 PARAM_DIR = os.path.join(HERE, "params")
 
 CK_MODDOWN, CK_RESCALE, CK_MODDOWN_RESCALE = 0, 1, 2
+CK_OPT_BCONV_MFMA = 1
 
 _vp = ctypes.c_void_p
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -84,6 +85,7 @@ _SIGS = {
     "ck_plan_run": (ctypes.c_int, [_vp, _vp]),
     "ck_plan_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
     "ck_plan_destroy": (ctypes.c_int, [_vp]),
+    "ck_context_set_option": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "ck_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ck_profile_report": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p,
                                          ctypes.POINTER(ctypes.c_double),
@@ -317,6 +319,13 @@ class Context:
                                    _ptrs([o[0] for o in outs]), _ptrs([o[1] for o in outs]),
                                    _ptr(ct[0]), _ptr(ct[1]), _ptrs(kb), _ptrs(ka),
                                    int(moddown), _stream(stream)))
+
+    def set_bconv_mfma(self, on=True):
+        """BaseConv on the FP64 matrix cores (True), fused into the forward
+        NTT column pass (False), or None for the default (matrix cores in
+        batched plans only); outputs are identical."""
+        v = -1 if on is None else int(bool(on))
+        _check(lib.ck_context_set_option(self._h, CK_OPT_BCONV_MFMA, v))
 
     # ---- per-kernel timing --------------------------------------------------
     def profile(self, on=True):

@@ -571,6 +571,18 @@ int ck_plan_destroy(ck_plan *plan) {
   return CK_OK;
 }
 
+int ck_context_set_option(ck_context *ctx, int option, int value) {
+  CK_REQUIRE(ctx && ctx->c, "null context");
+  switch (option) {
+    case CK_OPT_BCONV_MFMA:
+      CK_REQUIRE(value >= -1 && value <= 1, "option value must be -1, 0 or 1");
+      ctx->c->bconv_mfma = value;
+      return CK_OK;
+    default:
+      return fail(CK_ERR_INVALID_ARG, "unknown context option");
+  }
+}
+
 int ck_profile_enable(ck_context *ctx, int on) {
   CK_REQUIRE(ctx && ctx->c, "null context");
   prof_reset(*ctx->c);

@@ -152,6 +152,11 @@ struct Context {
 
   std::vector<LevelPlan> levels;
 
+  // BaseConv on the FP64 matrix cores (bconv_mfma) instead of fused into
+  // the forward column pass: 1 always, 0 never, -1 (default) for batched
+  // plans only, where it measured faster
+  int bconv_mfma = -1;
+
   // workspace for multi-kernel operations (one operation at a time per
   // context; concurrent calls on different streams race)
   uint32_t *ws = nullptr;

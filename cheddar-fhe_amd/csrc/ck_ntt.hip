@@ -276,7 +276,9 @@ __global__ __launch_bounds__(kColThreads, 5) void ntt_fwd_cols(
     for (int k = 0; k < 16; k++) x[k] = (double)lds[(rb + 16 * k) * kColTile + cc];
     __syncthreads();  // the staging words are reused by the double tile
   } else {
-    const uint32_t *__restrict__ g = J.src + c0 + cc;
+    // canonical residues (< 2^31) or a lazy signed BaseConv output: int32
+    const int32_t *__restrict__ g =
+        reinterpret_cast<const int32_t *>(J.src) + c0 + cc;
 #pragma unroll
     for (int k = 0; k < 16; k++) x[k] = (double)g[(rb + 16 * k) * 256];
   }

@@ -114,6 +114,17 @@ int ImmediateSink::copy(uint32_t *dst, const uint32_t *src, size_t words) {
                    "copy");
 }
 
+int ImmediateSink::bconv(const std::vector<BconvInst> &insts) {
+  for (size_t i = 0; i < insts.size(); i += kMaxBconvInst) {
+    BconvLaunch L;
+    std::memset(&L, 0, sizeof(L));
+    L.n = (int)std::min<size_t>(kMaxBconvInst, insts.size() - i);
+    for (int k = 0; k < L.n; k++) L.inst[k] = insts[i + k];
+    launch_bconv_mfma(c_, L, st_);
+  }
+  return launch_status("bconv");
+}
+
 int ImmediateSink::key_accum(const EwNp &np, const KeyAccumArgs &A) {
   launch_key_accum(c_, np, A, st_);
   return launch_status("key accumulate");
@@ -217,27 +228,48 @@ int op_modup(const Context &c, int level, uint32_t *dst, const uint32_t *src,
     int rc = s.pass(false, kInttCentred | (prod2 ? kInttLoadProduct : 0), jobs);
     if (rc) return rc;
   }
-  // 2. BaseConv of each group fused into the forward NTT of its outputs
+  // 2. BaseConv of each group, on the matrix cores or fused into the
+  // forward NTT of its outputs
+  const bool mfma = s.mfma_bconv();
   std::vector<PassJob> jobs;
+  std::vector<BconvInst> insts;
   for (int g = 0; g < lp.beta; g++) {
     const ModUpGroup &gr = lp.groups[g];
     if (gr.s1 <= gr.s0) continue;
     const int sl = gr.s1 - gr.s0;
+    const int32_t *bsrc = reinterpret_cast<const int32_t *>(v + (size_t)gr.s0 * kLimb);
+    if (mfma) {
+      BconvInst I;
+      std::memset(&I, 0, sizeof(I));
+      I.src = bsrc;
+      I.dst = dst;
+      I.tab = gr.d_ftab;
+      I.S = sl;
+      I.D = (int)gr.dst_limb.size();
+      for (int m = 0; m < I.D; m++) {
+        I.dst_limb[m] = (uint16_t)(g * tot + gr.dst_limb[m]);
+        I.dst_prime[m] = (uint8_t)gr.dst_prime[m];
+      }
+      insts.push_back(I);
+    }
     for (size_t m = 0; m < gr.dst_limb.size(); m++) {
       PassJob j;
       j.src = dst;
       j.dst = dst;
       j.src_limb = j.dst_limb = g * tot + gr.dst_limb[m];
-      j.bsrc = reinterpret_cast<const int32_t *>(v + (size_t)gr.s0 * kLimb);
-      j.btab = gr.d_ftab;
-      j.bs = sl;
-      j.btab_off = 2 * (int)m * sl;
+      if (!mfma) {
+        j.bsrc = bsrc;
+        j.btab = gr.d_ftab;
+        j.bs = sl;
+        j.btab_off = 2 * (int)m * sl;
+      }
       j.prime = gr.dst_prime[m];
       j.c1 = 1;
       jobs.push_back(j);
     }
   }
-  int rc = s.pass(true, kNttBconvInput, jobs);
+  int rc = mfma ? s.bconv(insts) : kOk;
+  if (!rc) rc = s.pass(true, mfma ? 0 : kNttBconvInput, jobs);
   if (rc) return rc;
   if (copy_own) {
     for (int g = 0; g < lp.beta; g++) {
@@ -287,9 +319,28 @@ int op_moddown(const Context &c, int level, int kind, int npoly,
     int rc = s.pass(false, kInttCentred, jobs);
     if (rc) return rc;
   }
-  // 2. BaseConv fused into the forward NTT, 3. epilogue in the row pass
+  // 2. BaseConv (matrix cores, or fused into the forward NTT), 3. epilogue
+  // in the row pass
+  const bool mfma = s.mfma_bconv();
   std::vector<PassJob> jobs;
-  for (int k = 0; k < npoly; k++)
+  std::vector<BconvInst> insts;
+  for (int k = 0; k < npoly; k++) {
+    const int32_t *bsrc =
+        reinterpret_cast<const int32_t *>(v + (size_t)k * mp->src_len * kLimb);
+    if (mfma) {
+      BconvInst I;
+      std::memset(&I, 0, sizeof(I));
+      I.src = bsrc;
+      I.dst = dst[k];
+      I.tab = mp->d_ftab;
+      I.S = mp->src_len;
+      I.D = mp->dst_len;
+      for (int y = 0; y < I.D; y++) {
+        I.dst_limb[y] = (uint16_t)y;
+        I.dst_prime[y] = (uint8_t)mp->dst_prime[y];
+      }
+      insts.push_back(I);
+    }
     for (int y = 0; y < mp->dst_len; y++) {
       PassJob j;
       j.src = dst[k];
@@ -297,18 +348,25 @@ int op_moddown(const Context &c, int level, int kind, int npoly,
       j.src_limb = j.dst_limb = y;
       j.src2 = src[k];
       j.src2_limb = mp->src2_limb[y];
-      j.bsrc = reinterpret_cast<const int32_t *>(v + (size_t)k * mp->src_len * kLimb);
-      j.btab = mp->d_ftab;
-      j.bs = mp->src_len;
-      j.btab_off = 2 * y * mp->src_len;
+      if (!mfma) {
+        j.bsrc = bsrc;
+        j.btab = mp->d_ftab;
+        j.bs = mp->src_len;
+        j.btab_off = 2 * y * mp->src_len;
+      }
       j.prime = mp->dst_prime[y];
       j.c1 = 1;
       j.c2 = mp->pad[y];
       j.c3 = mp->inv_prod[y];
       jobs.push_back(j);
     }
+  }
+  if (mfma) {
+    int rc = s.bconv(insts);
+    if (rc) return rc;
+  }
   return s.pass(true,
-                kNttBconvInput | kNttModDownEpilogue |
+                (mfma ? 0 : kNttBconvInput) | kNttModDownEpilogue |
                     (mp->use_pad ? kNttEpiloguePad : 0),
                 jobs);
 }

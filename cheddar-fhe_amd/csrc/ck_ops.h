@@ -7,6 +7,7 @@
 // merges each step across them into one launch.
 #pragma once
 
+#include "ck_bconv.h"
 #include "ck_elem.h"
 
 namespace ck {
@@ -34,6 +35,10 @@ class Sink {
   // forward (column + row pass) or inverse (row + column pass) transform
   virtual int pass(bool forward, int flags, const std::vector<PassJob> &jobs) = 0;
   virtual int copy(uint32_t *dst, const uint32_t *src, size_t words) = 0;
+  // BaseConv instances on the matrix cores (outputs: lazy int32 limbs)
+  virtual int bconv(const std::vector<BconvInst> &insts) = 0;
+  // whether ops should emit bconv() steps rather than fused BaseConv passes
+  virtual bool mfma_bconv() const = 0;
   virtual int key_accum(const EwNp &np, const KeyAccumArgs &A) = 0;
   virtual int bs_fused(const EwNp &np, const BsFusedArgs &A) = 0;
   virtual int permute(const EwNp &np, const EwOperand &x, const EwOperand &out,
@@ -47,6 +52,8 @@ class ImmediateSink : public Sink {
   ImmediateSink(const Context &c, hipStream_t st) : c_(c), st_(st) {}
   int pass(bool forward, int flags, const std::vector<PassJob> &jobs) override;
   int copy(uint32_t *dst, const uint32_t *src, size_t words) override;
+  int bconv(const std::vector<BconvInst> &insts) override;
+  bool mfma_bconv() const override { return c_.bconv_mfma == 1; }
   int key_accum(const EwNp &np, const KeyAccumArgs &A) override;
   int bs_fused(const EwNp &np, const BsFusedArgs &A) override;
   int permute(const EwNp &np, const EwOperand &x, const EwOperand &out,

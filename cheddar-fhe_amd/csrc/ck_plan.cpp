@@ -73,6 +73,12 @@ int PlanSink::copy(uint32_t *, const uint32_t *, size_t) {
   return kErrUnsupported;
 }
 
+int PlanSink::bconv(const std::vector<BconvInst> &insts) {
+  PlanStage &s = stage(PlanStage::kBconv);
+  s.bc.insert(s.bc.end(), insts.begin(), insts.end());
+  return kOk;
+}
+
 int PlanSink::key_accum(const EwNp &np, const KeyAccumArgs &A) {
   PlanStage &s = stage(PlanStage::kKeyAccum);
   const uint16_t item = (uint16_t)s.ka.size();
@@ -130,7 +136,7 @@ int plan_finish(Plan &p, bool mismatch) {
   }
   for (PlanStage &s : p.stages) {
     order_by_prime(s);
-    if (s.ka.size() > 65535 || s.bs.size() > 65535) {
+    if (s.ka.size() > 65535 || s.bs.size() > 65535 || s.bc.size() > 65535) {
       set_error("too many items in one plan step");
       return kErrUnsupported;
     }
@@ -138,6 +144,7 @@ int plan_finish(Plan &p, bool mismatch) {
     if (!rc) rc = upload_vec(p, &s.d_ka, s.ka);
     if (!rc) rc = upload_vec(p, &s.d_bs, s.bs);
     if (!rc) rc = upload_vec(p, &s.d_ew, s.ew);
+    if (!rc) rc = upload_vec(p, &s.d_bc, s.bc);
     if (rc) return rc;
   }
   return kOk;
@@ -170,6 +177,16 @@ int plan_run(const Plan &p, hipStream_t st) {
       case PlanStage::kBsFused:
         launch_bs_fused_table(c, s.d_bs, s.d_ew, (int)s.ew.size(), st);
         break;
+      case PlanStage::kBconv: {
+        int dsts = 0;
+        bool small = true;
+        for (const BconvInst &I : s.bc) {
+          dsts += I.D;
+          small = small && I.D <= 32 && I.S <= 12;
+        }
+        launch_bconv_mfma_table(c, s.d_bc, (int)s.bc.size(), dsts, small, st);
+        break;
+      }
     }
   }
   return check_hip(hipGetLastError(), "plan run");

@@ -8,13 +8,15 @@ namespace ck {
 // One step of a plan: a transform pass over every job's limb, or one
 // key-accumulation kernel over every item's limbs.
 struct PlanStage {
-  enum Kind { kFwd, kInv, kKeyAccum, kBsFused } kind = kFwd;
+  enum Kind { kFwd, kInv, kKeyAccum, kBsFused, kBconv } kind = kFwd;
   int flags_any = 0;
   std::vector<LimbJob> jobs;
   std::vector<KeyAccumItem> ka;
   std::vector<BsFusedItem> bs;
   std::vector<EwJob> ew;
+  std::vector<BconvInst> bc;
   LimbJob *d_jobs = nullptr;
+  BconvInst *d_bc = nullptr;
   KeyAccumItem *d_ka = nullptr;
   BsFusedItem *d_bs = nullptr;
   EwJob *d_ew = nullptr;
@@ -33,10 +35,12 @@ struct Plan {
 class PlanSink : public Sink {
  public:
   explicit PlanSink(Plan &p) : plan_(p) {}
+  bool mfma_bconv() const override { return plan_.ctx->bconv_mfma != 0; }
   void begin_item() { cursor_ = 0; }
   bool mismatch() const { return mismatch_; }
   int pass(bool forward, int flags, const std::vector<PassJob> &jobs) override;
   int copy(uint32_t *dst, const uint32_t *src, size_t words) override;
+  int bconv(const std::vector<BconvInst> &insts) override;
   int key_accum(const EwNp &np, const KeyAccumArgs &A) override;
   int bs_fused(const EwNp &np, const BsFusedArgs &A) override;
   int permute(const EwNp &np, const EwOperand &x, const EwOperand &out,

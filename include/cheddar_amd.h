@@ -251,6 +251,16 @@ int ck_plan_run(ck_plan *plan, void *stream);
 int ck_plan_info(const ck_plan *plan, long long *out);
 int ck_plan_destroy(ck_plan *plan);
 
+/* ---- Options ------------------------------------------------------------------ */
+/* CK_OPT_BCONV_MFMA: 1 runs every BaseConv (ModUp, ModDown, Rescale) on the
+ * FP64 matrix cores as its own step, 0 fuses it into the forward NTT's
+ * column pass, -1 (default) uses the matrix cores for batched plans only;
+ * results are identical. No reference counterpart (the
+ * reference's kFuseModDownEpilogue-style compile-time switches,
+ * include/core/Parameter.h:13-20, select kernel fusions the same way). */
+#define CK_OPT_BCONV_MFMA 1
+int ck_context_set_option(ck_context *ctx, int option, int value);
+
 /* ---- Per-kernel timing ---------------------------------------------------- */
 /* Replaces the wall-clock profiling macros of the reference's test harness
  * (unittest/Testbed.h:24-47) with device-side timing: while enabled, every

@@ -23,14 +23,20 @@ def ck():
     return cheddar_amd
 
 
-@pytest.fixture(scope="module")
-def ctx24(ck, rns24):
-    return ck.Context.from_json(rns24)
+# every test runs with BaseConv fused into the forward NTT column pass and
+# with BaseConv on the FP64 matrix cores: outputs must be identical
+@pytest.fixture(scope="module", params=["bconv-fused", "bconv-mfma"])
+def ctx24(ck, rns24, request):
+    ctx = ck.Context.from_json(rns24)
+    ctx.set_bconv_mfma(request.param == "bconv-mfma")
+    return ctx
 
 
-@pytest.fixture(scope="module")
-def ctx40(ck, boot40):
-    return ck.Context.from_json(boot40)
+@pytest.fixture(scope="module", params=["bconv-fused", "bconv-mfma"])
+def ctx40(ck, boot40, request):
+    ctx = ck.Context.from_json(boot40)
+    ctx.set_bconv_mfma(request.param == "bconv-mfma")
+    return ctx
 
 
 def sync():

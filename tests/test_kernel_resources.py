@@ -40,7 +40,7 @@ def resources(src):
 
 
 @pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
-@pytest.mark.parametrize("src", ["ck_ntt.hip", "ck_elem.hip"])
+@pytest.mark.parametrize("src", ["ck_ntt.hip", "ck_elem.hip", "ck_bconv.hip"])
 def test_no_spills_and_occupancy(src):
     res = resources(src)
     assert res, "no kernels reported"
