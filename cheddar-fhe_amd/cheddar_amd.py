@@ -54,6 +54,14 @@ _SIGS = {
                               ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
     "ck_mult_const": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, _vp]),
+    "ck_add_const": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, ctypes.c_int,
+                                    _vp]),
+    "ck_mult_imaginary_unit": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_int, ctypes.POINTER(_vp),
+                                              ctypes.POINTER(_vp), _vp]),
+    "ck_modup_to_max": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int,
+                                       _vp]),
     "ck_tensor": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int] + [_vp] * 7 + [_vp]),
     "ck_permute": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, _vp]),
@@ -266,6 +274,19 @@ class Context:
     def mult_const(self, dsts, xs, consts_dev, np3, stream=None):
         _check(lib.ck_mult_const(self._h, *np3, len(dsts), _ptrs(dsts), _ptrs(xs),
                                  _ptr(consts_dev), _stream(stream)))
+
+    def add_const(self, dsts, xs, consts_dev, np3, mode=0, stream=None):
+        """mode 0: x + c, 1: x - c, 2: c - x (c per limb, device array)"""
+        _check(lib.ck_add_const(self._h, *np3, len(dsts), _ptrs(dsts), _ptrs(xs),
+                                _ptr(consts_dev), mode, _stream(stream)))
+
+    def mult_imaginary_unit(self, dsts, xs, np3, stream=None):
+        _check(lib.ck_mult_imaginary_unit(self._h, *np3, len(dsts), _ptrs(dsts), _ptrs(xs),
+                                          _stream(stream)))
+
+    def modup_to_max(self, dst, src, base_np, num_aux=0, stream=None):
+        _check(lib.ck_modup_to_max(self._h, base_np[0], base_np[1], _ptr(src), _ptr(dst),
+                                   num_aux, _stream(stream)))
 
     def tensor(self, d0, d1, d2, b1, a1, b2, a2, num_main, num_ter, stream=None):
         _check(lib.ck_tensor(self._h, num_main, num_ter, _ptr(d0), _ptr(d1), _ptr(d2),

@@ -277,6 +277,60 @@ int ck_mult_const(ck_context *ctx, int num_main, int num_ter, int num_aux,
   return check_hip(hipGetLastError(), "mult_const");
 }
 
+int ck_add_const(ck_context *ctx, int num_main, int num_ter, int num_aux,
+                 int npoly, uint32_t *const *dst, const uint32_t *const *x,
+                 const uint32_t *consts, int mode, void *stream) {
+  CK_REQUIRE(ctx && ctx->c && consts, "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_np(c, num_main, num_ter, num_aux), "invalid NP");
+  CK_REQUIRE(npoly >= 1 && npoly <= kMaxPolys, "npoly must be 1..3");
+  CK_REQUIRE(mode >= 0 && mode <= 2, "mode must be 0 (add), 1 (sub) or 2 (c - x)");
+  CK_REQUIRE(valid_ptrs(dst, npoly) && valid_ptrs(x, npoly), "null pointer");
+  EwNp np = make_np(c, num_main, num_ter, num_aux);
+  EwOperand ox{}, od{};
+  for (int j = 0; j < npoly; j++) {
+    ox.ptr[j] = const_cast<uint32_t *>(x[j]);
+    od.ptr[j] = dst[j];
+  }
+  launch_const(c, mode + 1, np, ox, consts, 0, od, npoly, as_stream(stream));
+  return check_hip(hipGetLastError(), "add_const");
+}
+
+int ck_mult_imaginary_unit(ck_context *ctx, int num_main, int num_ter,
+                           int num_aux, int npoly, uint32_t *const *dst,
+                           const uint32_t *const *x, void *stream) {
+  CK_REQUIRE(ctx && ctx->c, "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_np(c, num_main, num_ter, num_aux), "invalid NP");
+  CK_REQUIRE(npoly >= 1 && npoly <= kMaxPolys, "npoly must be 1..3");
+  CK_REQUIRE(valid_ptrs(dst, npoly) && valid_ptrs(x, npoly), "null pointer");
+  EwNp np = make_np(c, num_main, num_ter, num_aux);
+  EwOperand ox{}, od{};
+  for (int j = 0; j < npoly; j++) {
+    ox.ptr[j] = const_cast<uint32_t *>(x[j]);
+    od.ptr[j] = dst[j];
+  }
+  launch_imag_unit(c, np, ox, od, npoly, as_stream(stream));
+  return check_hip(hipGetLastError(), "mult_imaginary_unit");
+}
+
+int ck_modup_to_max(ck_context *ctx, int base_main, int base_ter,
+                    const uint32_t *src, uint32_t *dst, int num_aux,
+                    void *stream) {
+  CK_REQUIRE(ctx && ctx->c && src && dst, "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_np(c, base_main, base_ter, 0), "invalid base NP");
+  const int nbase = base_main + base_ter;
+  CK_REQUIRE(nbase == 1 || nbase == 2, "the base must hold one or two primes");
+  CK_REQUIRE(num_aux >= 0 && num_aux <= c.alpha, "invalid num_aux");
+  CK_REQUIRE(dst != src, "dst must not alias src");
+  const int p0 = c.max_ter - base_ter;
+  const uint32_t q0 = c.primes[p0], q1 = nbase == 2 ? c.primes[p0 + 1] : 0u;
+  EwNp np = make_np(c, c.max_main, c.max_ter, num_aux);
+  launch_modup_to_max(c, np, src, nbase, q0, q1, dst, as_stream(stream));
+  return check_hip(hipGetLastError(), "modup_to_max");
+}
+
 int ck_tensor(ck_context *ctx, int num_main, int num_ter, uint32_t *d0,
               uint32_t *d1, uint32_t *d2, const uint32_t *b1,
               const uint32_t *a1, const uint32_t *b2, const uint32_t *a2,

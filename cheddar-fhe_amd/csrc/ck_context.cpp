@@ -380,7 +380,7 @@ int context_create(int log_n, const uint32_t *main_primes, int n_main,
 
   std::vector<int32_t> qinv(np);
   std::vector<double> qd(np), qr(np);
-  std::vector<uint32_t> tw_f((size_t)np * n), tw_i((size_t)np * n);
+  std::vector<uint32_t> tw_f((size_t)np * n), tw_i((size_t)np * n), imag(np);
   for (int i = 0; i < np; i++) {
     const uint32_t q = all[i];
     qinv[i] = inv_mod_2_32(q);
@@ -397,6 +397,7 @@ int context_create(int log_n, const uint32_t *main_primes, int n_main,
       pf = pf * psi % q;
       pi = pi * psi_inv % q;
     }
+    imag[i] = h_to_mont(h_powmod(psi, n / 2, q), q);
   }
   c->qinv = qinv;
   // P mod q_i in Montgomery form (Context.cpp:85-98)
@@ -420,6 +421,7 @@ int context_create(int log_n, const uint32_t *main_primes, int n_main,
   if (!rc) rc = upload(*c, &c->d_tw_fwd, tw_f);
   if (!rc) rc = upload(*c, &c->d_tw_inv, tw_i);
   if (!rc) rc = upload(*c, &c->d_p_prod, c->p_prod_mont);
+  if (!rc) rc = upload(*c, &c->d_imag, imag);
   if (rc) {
     context_destroy(c);
     return rc;

@@ -98,6 +98,11 @@ void launch_unary(const Context &ctx, int op, const EwNp &np,
 void launch_const(const Context &ctx, int op, const EwNp &np,
                   const EwOperand &x, const uint32_t *consts, int const_gap,
                   const EwOperand &out, int npoly, hipStream_t st);
+void launch_imag_unit(const Context &ctx, const EwNp &np, const EwOperand &x,
+                      const EwOperand &out, int npoly, hipStream_t st);
+void launch_modup_to_max(const Context &ctx, const EwNp &np, const uint32_t *src,
+                         int nbase, uint32_t q0, uint32_t q1, uint32_t *dst,
+                         hipStream_t st);
 void launch_tensor(const Context &ctx, const EwNp &np, const EwOperand &c1,
                    const EwOperand &c2, const EwOperand &out, hipStream_t st);
 void launch_key_accum(const Context &ctx, const EwNp &np,

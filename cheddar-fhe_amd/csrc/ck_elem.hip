@@ -80,7 +80,8 @@ __global__ __launch_bounds__(kThreads) void unary_kernel(const EwNp np,
 }
 
 // CPAccum<const> with one term = MultConst: x * c_y (c in Montgomery form)
-// op: 0 mult, 1 add (c + x), 2 sub (x - c)
+// op: 0 mult, 1 add (c + x), 2 sub (x - c), 3 opposite sub (c - x)
+// (AddConst / SubConst / SubOppositeConst, ElementWise.cu:145-223)
 template <int OP>
 __global__ __launch_bounds__(kThreads) void const_kernel(
     const EwNp np, const PrimeTab P, EwOperand x, const uint32_t *consts,
@@ -94,10 +95,58 @@ __global__ __launch_bounds__(kThreads) void const_kernel(
       r = map2(a, a, [&](uint32_t u, uint32_t) { return mont_mul(u, c, q, qinv); });
     else if (OP == 1)
       r = map2(a, a, [&](uint32_t u, uint32_t) { return add_mod(u, c, q); });
-    else
+    else if (OP == 2)
       r = map2(a, a, [&](uint32_t u, uint32_t) { return sub_mod(u, c, q); });
+    else
+      r = map2(a, a, [&](uint32_t u, uint32_t) { return sub_mod(c, u, q); });
     st4(out.ptr[j] + ew_off(out, np, y, k), r);
   }
+}
+
+// MultImaginaryUnit (ElementWise.cu:625-658): multiplication by X^(N/2).
+// NTT position k holds the value at psi^(2 brv(k) + 1), so the factor is
+// psi^(N/2) (-1)^brv(k), and brv(k) is odd exactly for k >= N/2. imag[p]
+// is psi_p^(N/2) in Montgomery form.
+__global__ __launch_bounds__(kThreads) void imag_unit_kernel(
+    const EwNp np, const PrimeTab P, const uint32_t *imag, EwOperand x,
+    EwOperand out, int npoly) {
+  EW_PROLOGUE
+  const uint32_t c0 = imag[p];
+  const uint32_t c = k >= 32768 ? q - c0 : c0;
+  for (int j = 0; j < npoly; j++) {
+    uint4 a = ld4(x.ptr[j] + ew_off(x, np, y, k));
+    uint4 r = map2(a, a, [&](uint32_t u, uint32_t) { return mont_mul(u, c, q, qinv); });
+    st4(out.ptr[j] + ew_off(out, np, y, k), r);
+  }
+}
+
+// ModUpToMax1 / ModUpToMax2 (ElementWise.cu:579-623): coefficient-domain
+// limbs of a one- or two-prime base lifted to every destination prime.
+// Two primes: v = s0 q1 + s1 q0 in [0, 2 q0 q1), centred into
+// (-q0 q1 / 2, q0 q1 / 2], then reduced; one prime: s0 mod p (no centring).
+__global__ __launch_bounds__(kThreads) void modup_to_max_kernel(
+    const EwNp np, const PrimeTab P, const uint32_t *src, int nbase,
+    uint32_t q0, uint32_t q1, uint32_t *dst) {
+  EW_PROLOGUE
+  const uint4 s0 = ld4(src + k);
+  uint4 r;
+  if (nbase == 2) {
+    const uint4 s1 = ld4(src + 65536 + k);
+    const int64_t qq = (int64_t)q0 * q1, half = qq >> 1;
+    auto lift = [&](uint32_t u, uint32_t v) {
+      int64_t t = (int64_t)u * q1 + (int64_t)v * q0;
+      if (t >= qq) t -= qq;
+      if (t > half) t -= qq;
+      int64_t m = t % (int64_t)q;
+      if (m < 0) m += q;
+      return (uint32_t)m;
+    };
+    r = make_uint4(lift(s0.x, s1.x), lift(s0.y, s1.y), lift(s0.z, s1.z),
+                   lift(s0.w, s1.w));
+  } else {
+    r = make_uint4(s0.x % q, s0.y % q, s0.z % q, s0.w % q);
+  }
+  st4(dst + (size_t)y * 65536 + k, r);
 }
 
 // Tensor (ElementWise.cu:504-546): (b1 b2, (b1+a1)(b2+a2) - b1 b2 - a1 a2,
@@ -383,9 +432,28 @@ void launch_const(const Context &ctx, int op, const EwNp &np,
   else if (op == 1)
     hipLaunchKernelGGL(const_kernel<1>, ew_grid(np), dim3(kThreads), 0, st, np,
                        ctx.ptab, x, consts, const_gap, out, npoly);
-  else
+  else if (op == 2)
     hipLaunchKernelGGL(const_kernel<2>, ew_grid(np), dim3(kThreads), 0, st, np,
                        ctx.ptab, x, consts, const_gap, out, npoly);
+  else
+    hipLaunchKernelGGL(const_kernel<3>, ew_grid(np), dim3(kThreads), 0, st, np,
+                       ctx.ptab, x, consts, const_gap, out, npoly);
+}
+
+void launch_imag_unit(const Context &ctx, const EwNp &np, const EwOperand &x,
+                      const EwOperand &out, int npoly, hipStream_t st) {
+  if (np.num_total <= 0) return;
+  launch(ctx, "mult_imaginary_unit", np.num_total, imag_unit_kernel, ew_grid(np),
+         dim3(kThreads), st, np, ctx.ptab, (const uint32_t *)ctx.d_imag, x, out,
+         npoly);
+}
+
+void launch_modup_to_max(const Context &ctx, const EwNp &np, const uint32_t *src,
+                         int nbase, uint32_t q0, uint32_t q1, uint32_t *dst,
+                         hipStream_t st) {
+  if (np.num_total <= 0) return;
+  launch(ctx, "modup_to_max", np.num_total, modup_to_max_kernel, ew_grid(np),
+         dim3(kThreads), st, np, ctx.ptab, src, nbase, q0, q1, dst);
 }
 
 void launch_tensor(const Context &ctx, const EwNp &np, const EwOperand &c1,

@@ -148,6 +148,7 @@ struct Context {
   uint32_t *d_tw_fwd = nullptr;  // psi^brv(j), plain, per prime
   uint32_t *d_tw_inv = nullptr;  // psi^-brv(j), plain, per prime
   uint32_t *d_p_prod = nullptr;  // P mod q * R, indexed by global q prime
+  uint32_t *d_imag = nullptr;    // psi^(N/2) * R (X^(N/2) in NTT form), per prime
   PrimeTab ptab{};
 
   std::vector<LevelPlan> levels;

@@ -176,10 +176,40 @@ inline std::unique_ptr<Parameter<uint32_t>> ParameterFromJsonText(
     levels.emplace_back((int)Int(e.arr[0], "level_config"),
                         (int)Int(e.arr[1], "level_config"));
   }
-  return std::make_unique<Parameter<uint32_t>>(
+  std::pair<int, int> additional_base{0, 0};
+  auto ab = root.obj.find("additional_base");
+  if (ab != root.obj.end()) {
+    if (ab->second.kind != Value::kArray || ab->second.arr.size() != 2)
+      throw Error("additional_base should be an array of pairs");
+    additional_base = {(int)Int(ab->second.arr[0], "additional_base"),
+                       (int)Int(ab->second.arr[1], "additional_base")};
+  }
+  auto param = std::make_unique<Parameter<uint32_t>>(
       log_degree, std::ldexp(1.0, log_scale), enc_level, levels,
       Primes(root, "main_primes", true), Primes(root, "auxiliary_primes", true),
-      Primes(root, "terminal_primes", false));
+      Primes(root, "terminal_primes", false), additional_base);
+  auto dh = root.obj.find("dense_hamming_weight");
+  if (dh != root.obj.end())
+    param->SetDenseHammingWeight((int)Int(dh->second, "dense_hamming_weight"));
+  auto sh = root.obj.find("sparse_hamming_weight");
+  if (sh != root.obj.end())
+    param->SetSparseHammingWeight((int)Int(sh->second, "sparse_hamming_weight"));
+  return param;
+}
+
+// Bootstrapping fields of the same JSON ("boot", "num_cts_levels",
+// "num_stc_levels", read as the reference's test bed does,
+// unittest/Testbed.h): returns false when "boot" is absent or false.
+inline bool BootLevelsFromJsonText(const std::string &text, int *num_cts_levels,
+                                   int *num_stc_levels) {
+  using namespace json_detail;
+  const Value root = Reader(text).Parse();
+  auto b = root.obj.find("boot");
+  if (b == root.obj.end() || b->second.kind != Value::kBool || !b->second.b)
+    return false;
+  *num_cts_levels = (int)Int(Field(root, "num_cts_levels"), "num_cts_levels");
+  *num_stc_levels = (int)Int(Field(root, "num_stc_levels"), "num_stc_levels");
+  return true;
 }
 
 inline std::unique_ptr<Parameter<uint32_t>> ParameterFromJsonFile(

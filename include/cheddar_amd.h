@@ -149,6 +149,24 @@ int ck_neg(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly,
 int ck_mult_const(ck_context *ctx, int num_main, int num_ter, int num_aux,
                   int npoly, uint32_t *const *dst, const uint32_t *const *x,
                   const uint32_t *consts, void *stream);
+/* ElementWiseHandler::AddConst / SubConst / SubOppositeConst
+ * (ElementWise.cu:145-223): mode 0: x + c[limb], 1: x - c[limb],
+ * 2: c[limb] - x; consts is a DEVICE array of num_total values */
+int ck_add_const(ck_context *ctx, int num_main, int num_ter, int num_aux,
+                 int npoly, uint32_t *const *dst, const uint32_t *const *x,
+                 const uint32_t *consts, int mode, void *stream);
+/* ElementWiseHandler::MultImaginaryUnit (ElementWise.cu:625-658, :1673-1703):
+ * x * X^(N/2) in the NTT domain (the slot-wise product by sqrt(-1)) */
+int ck_mult_imaginary_unit(ck_context *ctx, int num_main, int num_ter,
+                           int num_aux, int npoly, uint32_t *const *dst,
+                           const uint32_t *const *x, void *stream);
+/* ElementWiseHandler::ModUpToMax (ElementWise.cu:579-623, :1646-1671):
+ * src = the coefficient-domain limbs of the base NP (base_main, base_ter),
+ * one or two primes; dst = every main and terminal limb of the top level
+ * followed by num_aux auxiliary limbs (coefficient domain, [0, q)). */
+int ck_modup_to_max(ck_context *ctx, int base_main, int base_ter,
+                    const uint32_t *src, uint32_t *dst, int num_aux,
+                    void *stream);
 /* ElementWiseHandler::Tensor (ElementWise.cu:921-953, kernel :504-546):
  * (d0, d1, d2) = (b1 b2, b1 a2 + a1 b2, a1 a2) on num_q limbs */
 int ck_tensor(ck_context *ctx, int num_main, int num_ter, uint32_t *d0,

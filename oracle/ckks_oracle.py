@@ -294,6 +294,44 @@ class Param:
         lib.or_mult_const(self._h, _ints(pidx), len(pidx), _p(x), _p(c), _p(out))
         return out
 
+    def add_const(self, x, consts, pidx, mode=0):
+        """AddConst / SubConst / SubOppositeConst (src/core/ElementWise.cu:145-223):
+        mode 0: x + c, 1: x - c, 2: c - x, c one value per limb, in [0, q)."""
+        q = np.array([self.prime(i) for i in pidx], dtype=np.int64)[:, None]
+        xv = np.asarray(x, dtype=np.int64)
+        c = np.asarray(consts, dtype=np.int64)[:, None]
+        r = (xv + c) if mode == 0 else (xv - c) if mode == 1 else (c - xv)
+        return (r % q).astype(np.uint32)
+
+    def mult_imaginary_unit(self, x, pidx):
+        """MultImaginaryUnit (src/core/ElementWise.cu:625-658): NTT position k
+        times psi^(N/2), negated for k >= N/2 (the NTT form of X^(N/2))."""
+        out = np.empty(np.shape(x), dtype=np.uint32)
+        half = self.n // 2
+        for t, pi in enumerate(pidx):
+            q = self.prime(pi)
+            i_unit = pow(self.psi(pi), half, q)
+            row = np.asarray(x[t], dtype=np.int64)
+            out[t, :half] = (row[:half] * i_unit) % q
+            out[t, half:] = (row[half:] * (q - i_unit)) % q
+        return out
+
+    def modup_to_max(self, src, base_pidx, dst_pidx):
+        """ModUpToMax1/2 (src/core/ElementWise.cu:579-623): coefficient-domain
+        limbs of a one- or two-prime base lifted to every destination prime.
+        Two primes: v = s0 q1 + s1 q0, brought from [0, 2 q0 q1) into
+        (-q0 q1 / 2, q0 q1 / 2]; one prime: s0 mod t (not centred)."""
+        s = np.asarray(src, dtype=np.int64)
+        qd = np.array([self.prime(i) for i in dst_pidx], dtype=np.int64)[:, None]
+        if len(base_pidx) == 2:
+            q0, q1 = self.prime(base_pidx[0]), self.prime(base_pidx[1])
+            qq = q0 * q1  # < 2^62, and v < 2^63: int64 holds every value
+            v = s[0] * q1 + s[1] * q0
+            v = np.where(v >= qq, v - qq, v)
+            v = np.where(v > qq // 2, v - qq, v)
+            return (v[None, :] % qd).astype(np.uint32)
+        return (s[0][None, :] % qd).astype(np.uint32)
+
     def tensor(self, b1, a1, b2, a2, pidx):
         ins = [np.ascontiguousarray(a, dtype=np.uint32) for a in (b1, a1, b2, a2)]
         outs = [np.empty(ins[0].shape, dtype=np.uint32) for _ in range(3)]

@@ -1,9 +1,10 @@
 """The C++ host facade (cheddar-fhe_amd/host/include/cheddar/, the
-reference's Parameter / Context / Ciphertext / EvaluationKey API) driven by
-tests/cpp/test_context.cpp: parameter JSON in the reference's schema,
-HMult / Relinearize(Rescale) / Rescale / HRot / HConj / hoisted HRot /
-MultKeyNoModDown / Add / Sub / Neg on the GPU, each output compared word
-for word with the C oracle inside that program."""
+reference's Parameter / Context / Ciphertext / EvaluationKey / Encoder /
+UserInterface API) driven by tests/cpp/test_context.cpp (parameter JSON in
+the reference's schema, HMult / Relinearize(Rescale) / Rescale / HRot /
+HConj / hoisted HRot / MultKeyNoModDown / Add / Sub / Neg on the GPU, each
+output compared word for word with the C oracle inside that program) and
+tests/cpp/test_basic.cpp (encrypted semantics)."""
 import os
 import subprocess
 
@@ -14,6 +15,7 @@ from conftest import GOLDEN, PKG, REPO
 pytestmark = pytest.mark.gpu
 
 BIN = os.path.join(REPO, "tests", "cpp", "test_context")
+BASIC = os.path.join(REPO, "tests", "cpp", "test_basic")
 
 
 @pytest.mark.parametrize("params,level", [
@@ -30,3 +32,21 @@ def test_cpp_facade(params, level):
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "OK: 0 failure(s)" in r.stdout
     assert "FAIL" not in r.stdout
+
+
+# Encrypted semantics (unittest/BasicTest.cpp): encode, encrypt, evaluate,
+# decrypt, decode and compare with the plain computation (max error 1e-3).
+# bootparam_40 is run with a Hamming-weight-32 main secret (no sparse-secret
+# encapsulation), the setting the bootstrapping tests use.
+@pytest.mark.parametrize("params,weight", [
+    (os.path.join(PKG, "params", "rns24_aux8.json"), None),
+    (os.path.join(GOLDEN, "params", "bootparam_40.json"), 32),
+])
+def test_cpp_basic_semantics(params, weight):
+    assert os.path.exists(BASIC), "tests/cpp/test_basic is not built (run build())"
+    print(f"basic semantics {os.path.basename(params)} weight {weight}", flush=True)
+    args = [BASIC, params] + ([str(weight)] if weight else [])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=600)
+    print(r.stdout[-6000:])
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    assert "OK: 0 failure(s)" in r.stdout

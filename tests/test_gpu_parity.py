@@ -184,6 +184,53 @@ def test_elementwise(ck, ctx24, oracle_rns24):
     assert_same(ck.to_host(out), P.mult_const(x, consts, pidx), "mult_const")
 
 
+def test_const_ops_and_imaginary_unit(ck, ctx24, oracle_rns24):
+    P = oracle_rns24
+    pidx = P.np_primes(24, 0, 8)
+    np3 = (24, 0, 8)
+    x = uniform(P, pidx, SEED_BASE + 80)
+    y = uniform(P, pidx, SEED_BASE + 81)
+    rng = np.random.default_rng(SEED_BASE + 82)
+    c = np.array([rng.integers(0, P.prime(i)) for i in pidx], dtype=np.uint32)
+    c[0], c[1] = 0, P.prime(pidx[1]) - 1
+    dx, dy = ck.to_device(x), ck.to_device(y)
+    dc = ck.to_device(c[None, :])[0]
+    o1, o2 = torch.empty_like(dx), torch.empty_like(dx)
+    for mode in range(3):
+        ctx24.add_const([o1, o2], [dx, dy], dc, np3, mode=mode)
+        sync()
+        assert_same(ck.to_host(o1), P.add_const(x, c, pidx, mode), f"add_const mode {mode} x")
+        assert_same(ck.to_host(o2), P.add_const(y, c, pidx, mode), f"add_const mode {mode} y")
+    ctx24.mult_imaginary_unit([o1, o2], [dx, dy], np3)
+    sync()
+    assert_same(ck.to_host(o1), P.mult_imaginary_unit(x, pidx), "imaginary unit x")
+    assert_same(ck.to_host(o2), P.mult_imaginary_unit(y, pidx), "imaginary unit y")
+
+
+@pytest.mark.parametrize("num_aux", [0, 12])
+def test_boot40_modup_to_max(ck, ctx40, oracle_boot40, num_aux):
+    P = oracle_boot40
+    base = P.np_primes(0, 2)
+    src = uniform(P, base, SEED_BASE + 90)
+    dst = P.np_primes(*P.level_np(P.max_level), num_aux)
+    want = P.modup_to_max(src, base, dst)
+    out = torch.empty((len(dst), P.n), dtype=torch.int32, device="cuda")
+    ctx40.modup_to_max(out, ck.to_device(src), (0, 2), num_aux)
+    sync()
+    assert_same(ck.to_host(out), want, f"modup_to_max aux {num_aux}")
+
+
+def test_modup_to_max_single_prime(ck, ctx24, oracle_rns24):
+    P = oracle_rns24
+    base = P.np_primes(1, 0)
+    src = uniform(P, base, SEED_BASE + 91)
+    dst = P.np_primes(24, 0, 8)
+    out = torch.empty((len(dst), P.n), dtype=torch.int32, device="cuda")
+    ctx24.modup_to_max(out, ck.to_device(src), (1, 0), 8)
+    sync()
+    assert_same(ck.to_host(out), P.modup_to_max(src, base, dst), "modup_to_max one prime")
+
+
 def test_tensor(ck, ctx24, oracle_rns24):
     P = oracle_rns24
     pidx = P.level_primes(23)
