@@ -62,6 +62,12 @@ _SIGS = {
                                               ctypes.POINTER(_vp), _vp]),
     "ck_modup_to_max": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int,
                                        _vp]),
+    "ck_paccum": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                 ctypes.POINTER(_vp), ctypes.c_int, _vp]),
+    "ck_caccum": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                 ctypes.POINTER(_vp), ctypes.c_int, _vp]),
     "ck_tensor": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int] + [_vp] * 7 + [_vp]),
     "ck_permute": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, _vp]),
@@ -287,6 +293,17 @@ class Context:
     def modup_to_max(self, dst, src, base_np, num_aux=0, stream=None):
         _check(lib.ck_modup_to_max(self._h, base_np[0], base_np[1], _ptr(src), _ptr(dst),
                                    num_aux, _stream(stream)))
+
+    def paccum(self, dsts, xs, ys, np3, accumulate=False, stream=None):
+        """dsts[p] (+)= sum_t xs[t][p] * ys[t]; xs: list of per-term lists of polys"""
+        flat = [v for term in xs for v in term]
+        _check(lib.ck_paccum(self._h, *np3, len(dsts), len(ys), _ptrs(dsts), _ptrs(flat),
+                             _ptrs(ys), int(accumulate), _stream(stream)))
+
+    def caccum(self, dsts, xs, consts, np3, accumulate=False, stream=None):
+        flat = [v for term in xs for v in term]
+        _check(lib.ck_caccum(self._h, *np3, len(dsts), len(consts), _ptrs(dsts), _ptrs(flat),
+                             _ptrs(consts), int(accumulate), _stream(stream)))
 
     def tensor(self, d0, d1, d2, b1, a1, b2, a2, num_main, num_ter, stream=None):
         _check(lib.ck_tensor(self._h, num_main, num_ter, _ptr(d0), _ptr(d1), _ptr(d2),

@@ -331,6 +331,51 @@ int ck_modup_to_max(ck_context *ctx, int base_main, int base_ter,
   return check_hip(hipGetLastError(), "modup_to_max");
 }
 
+static int accum_common(ck_context *ctx, int num_main, int num_ter, int num_aux,
+                        int npoly, int nterms, uint32_t *const *dst,
+                        const uint32_t *const *x, const uint32_t *const *y,
+                        int y_const, int accumulate, void *stream) {
+  CK_REQUIRE(ctx && ctx->c && dst && (nterms == 0 || (x && y)), "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_np(c, num_main, num_ter, num_aux), "invalid NP");
+  CK_REQUIRE(npoly >= 1 && npoly <= 2, "npoly must be 1 or 2");
+  CK_REQUIRE(nterms >= 0, "nterms must not be negative");
+  CK_REQUIRE(valid_ptrs(dst, npoly), "null pointer");
+  CK_REQUIRE(nterms == 0 || (valid_ptrs(x, nterms * npoly) && valid_ptrs(y, nterms)),
+             "null pointer");
+  EwNp np = make_np(c, num_main, num_ter, num_aux);
+  int done = 0;
+  do {
+    AccumArgs A{};
+    A.nterms = std::min(nterms - done, kMaxAccTerms);
+    A.npoly = npoly;
+    A.y_const = y_const;
+    A.accumulate = accumulate || done > 0;
+    for (int j = 0; j < npoly; j++) A.dst[j] = dst[j];
+    for (int t = 0; t < A.nterms; t++) {
+      for (int j = 0; j < npoly; j++) A.x[t][j] = x[(size_t)(done + t) * npoly + j];
+      A.y[t] = y[done + t];
+    }
+    launch_accum(c, np, A, as_stream(stream));
+    done += A.nterms;
+  } while (done < nterms);
+  return check_hip(hipGetLastError(), y_const ? "caccum" : "paccum");
+}
+
+int ck_paccum(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly,
+              int nterms, uint32_t *const *dst, const uint32_t *const *x,
+              const uint32_t *const *y, int accumulate, void *stream) {
+  return accum_common(ctx, num_main, num_ter, num_aux, npoly, nterms, dst, x, y, 0,
+                      accumulate, stream);
+}
+
+int ck_caccum(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly,
+              int nterms, uint32_t *const *dst, const uint32_t *const *x,
+              const uint32_t *const *consts, int accumulate, void *stream) {
+  return accum_common(ctx, num_main, num_ter, num_aux, npoly, nterms, dst, x, consts, 1,
+                      accumulate, stream);
+}
+
 int ck_tensor(ck_context *ctx, int num_main, int num_ter, uint32_t *d0,
               uint32_t *d1, uint32_t *d2, const uint32_t *b1,
               const uint32_t *a1, const uint32_t *b2, const uint32_t *a2,

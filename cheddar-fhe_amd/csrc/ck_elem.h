@@ -74,6 +74,25 @@ struct BsFusedArgs {
   uint32_t ginv[kMaxBsRot];
 };
 
+// Sum of products over up to kMaxAccTerms terms for npoly (1..2)
+// polynomials sharing the multiplier of each term (PAccum / CAccum,
+// ElementWise.cu:225-366, and the giant-step GSFusedKernel,
+// src/extension/Hoist.cu:80-118):
+//   dst[p] = (accumulate ? dst[p] : 0) + sum_t x[t][p] (*) y[t]
+// where y[t] is a polynomial (per coefficient) or, with y_const, one value
+// per limb. All operands hold num_total contiguous limbs.
+constexpr int kMaxAccTerms = 16;
+
+struct AccumArgs {
+  const uint32_t *x[kMaxAccTerms][2];
+  const uint32_t *y[kMaxAccTerms];
+  uint32_t *dst[2];
+  int nterms;
+  int npoly;
+  int y_const;
+  int accumulate;
+};
+
 // Batched (table) forms: item i of a plan stage has its own layout and
 // arguments; job j covers limb jobs[j].y of item jobs[j].item.
 struct EwJob {
@@ -103,6 +122,8 @@ void launch_imag_unit(const Context &ctx, const EwNp &np, const EwOperand &x,
 void launch_modup_to_max(const Context &ctx, const EwNp &np, const uint32_t *src,
                          int nbase, uint32_t q0, uint32_t q1, uint32_t *dst,
                          hipStream_t st);
+void launch_accum(const Context &ctx, const EwNp &np, const AccumArgs &A,
+                  hipStream_t st);
 void launch_tensor(const Context &ctx, const EwNp &np, const EwOperand &c1,
                    const EwOperand &c2, const EwOperand &out, hipStream_t st);
 void launch_key_accum(const Context &ctx, const EwNp &np,

@@ -120,6 +120,32 @@ __global__ __launch_bounds__(kThreads) void imag_unit_kernel(
   }
 }
 
+// Sum of Montgomery products (see AccumArgs). Each product is canonical
+// and the sum is reduced after every term, so the result is the canonical
+// residue whatever the order, as the reference's kernels produce it.
+__global__ __launch_bounds__(kThreads) void accum_kernel(const EwNp np,
+                                                         const PrimeTab P,
+                                                         const AccumArgs A) {
+  EW_PROLOGUE
+  const size_t off = (size_t)y * 65536 + k;
+  for (int j = 0; j < A.npoly; j++) {
+    uint4 acc = A.accumulate ? ld4(A.dst[j] + off) : make_uint4(0, 0, 0, 0);
+    for (int t = 0; t < A.nterms; t++) {
+      const uint4 a = ld4(A.x[t][j] + off);
+      uint4 prod;
+      if (A.y_const) {
+        const uint32_t c = A.y[t][y];
+        prod = map2(a, a, [&](uint32_t u, uint32_t) { return mont_mul(u, c, q, qinv); });
+      } else {
+        prod = map2(a, ld4(A.y[t] + off),
+                    [&](uint32_t u, uint32_t v) { return mont_mul(u, v, q, qinv); });
+      }
+      acc = map2(acc, prod, [&](uint32_t u, uint32_t v) { return add_mod(u, v, q); });
+    }
+    st4(A.dst[j] + off, acc);
+  }
+}
+
 // ModUpToMax1 / ModUpToMax2 (ElementWise.cu:579-623): coefficient-domain
 // limbs of a one- or two-prime base lifted to every destination prime.
 // Two primes: v = s0 q1 + s1 q0 in [0, 2 q0 q1), centred into
@@ -446,6 +472,13 @@ void launch_imag_unit(const Context &ctx, const EwNp &np, const EwOperand &x,
   launch(ctx, "mult_imaginary_unit", np.num_total, imag_unit_kernel, ew_grid(np),
          dim3(kThreads), st, np, ctx.ptab, (const uint32_t *)ctx.d_imag, x, out,
          npoly);
+}
+
+void launch_accum(const Context &ctx, const EwNp &np, const AccumArgs &A,
+                  hipStream_t st) {
+  if (np.num_total <= 0) return;
+  launch(ctx, A.y_const ? "caccum" : "paccum", np.num_total, accum_kernel, ew_grid(np),
+         dim3(kThreads), st, np, ctx.ptab, A);
 }
 
 void launch_modup_to_max(const Context &ctx, const EwNp &np, const uint32_t *src,

@@ -167,6 +167,20 @@ int ck_mult_imaginary_unit(ck_context *ctx, int num_main, int num_ter,
 int ck_modup_to_max(ck_context *ctx, int base_main, int base_ter,
                     const uint32_t *src, uint32_t *dst, int num_aux,
                     void *stream);
+/* ElementWiseHandler::PAccum (ElementWise.cu:1437-1643, CPAccum kernels
+ * :225-366) and the giant-step accumulation GSFusedKernel
+ * (src/extension/Hoist.cu:80-118): for p < npoly (1 or 2),
+ *   dst[p] = (accumulate ? dst[p] : 0) + sum_t x[t * npoly + p] (*) y[t]
+ * with y[t] a polynomial; every operand holds the NP's limbs contiguously.
+ * Any nterms >= 0 (nterms = 0 clears dst unless accumulating). */
+int ck_paccum(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly,
+              int nterms, uint32_t *const *dst, const uint32_t *const *x,
+              const uint32_t *const *y, int accumulate, void *stream);
+/* ElementWiseHandler::CAccum (same kernels, constant multipliers): as
+ * ck_paccum with consts[t] a DEVICE array of one value per limb. */
+int ck_caccum(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly,
+              int nterms, uint32_t *const *dst, const uint32_t *const *x,
+              const uint32_t *const *consts, int accumulate, void *stream);
 /* ElementWiseHandler::Tensor (ElementWise.cu:921-953, kernel :504-546):
  * (d0, d1, d2) = (b1 b2, b1 a2 + a1 b2, a1 a2) on num_q limbs */
 int ck_tensor(ck_context *ctx, int num_main, int num_ter, uint32_t *d0,

@@ -1,0 +1,182 @@
+// Bootstrapping through the C++ host facade on the GPU (the reference's
+// unittest/Bootstrapping.cpp: encrypt a random message at level 0,
+// bootstrap, decrypt and compare within 1e-3), with stage checks first:
+// - EvalMod on values (I + r) / K against the plain evaluation of the same
+//   schedule and against sin(2 pi r) / (2 pi);
+// - the first CoeffToSlot and SlotToCoeff phases against the plain
+//   baby-step / giant-step product of their plaintext maps;
+// - the full bootstrap: precision bits = -log2(max error) and the mean
+//   error, and the wall time per bootstrap (after one warm-up).
+//
+//   test_boot <bootparam.json> [timed_runs]
+//
+// The main secret has Hamming weight equal to the sparse weight of the
+// parameter file (no sparse-secret encapsulation).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "cheddar/ParameterJson.h"
+#include "cheddar/UserInterface.h"
+#include "cheddar/extension/BootContext.h"
+
+using namespace cheddar;
+using word = uint32_t;
+using Ct = Ciphertext<word>;
+using Pt = Plaintext<word>;
+using Msg = std::vector<Complex>;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+int g_fail = 0;
+
+double Seconds(Clock::time_point t0) {
+  return std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
+void Report(const std::string &name, const Msg &want, const Msg &got, double tol) {
+  double worst = 0, sum = 0;
+  bool ok = want.size() == got.size() && !want.empty();
+  for (size_t i = 0; ok && i < want.size(); i++) {
+    const double e = std::abs(want[i] - got[i]);
+    if (!(e <= worst)) worst = e;
+    sum += e;
+  }
+  ok = ok && worst < tol;
+  std::printf("%s %s: max error %.3e (%.2f bits), mean error %.3e (%.2f bits)\n",
+              ok ? "PASS" : "FAIL", name.c_str(), worst, -std::log2(worst),
+              sum / want.size(), -std::log2(sum / want.size()));
+  std::fflush(stdout);
+  if (!ok) g_fail++;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: %s <bootparam.json> [timed_runs]\n", argv[0]);
+    return 2;
+  }
+  const int runs = argc > 2 ? std::atoi(argv[2]) : 2;
+  std::ifstream f(argv[1]);
+  std::stringstream ss;
+  ss << f.rdbuf();
+  const std::string text = ss.str();
+  auto param = ParameterFromJsonText(text);
+  int ncts = 0, nstc = 0;
+  if (!BootLevelsFromJsonText(text, &ncts, &nstc)) {
+    std::fprintf(stderr, "%s does not enable bootstrapping\n", argv[1]);
+    return 2;
+  }
+  param->SetDenseHammingWeight(param->GetSparseHammingWeight());
+  const Parameter<word> &P = *param;
+  const int slots = P.degree_ / 2;
+
+  auto t0 = Clock::now();
+  auto ctx = BootContext<word>::Create(P, BootParameter(P.max_level_, ncts, nstc));
+  UserInterface<word> ui(ctx, 0x4348454444415204ull);
+  ctx->PrepareEvalMod();
+  ctx->PrepareEvalSpecialFFT(slots);
+  EvkRequest req;
+  ctx->AddRequiredRotations(req, slots);
+  ui.PrepareRotationKey(req);
+  (void)hipDeviceSynchronize();
+  std::printf("setup: %.1f s (%zu rotation keys; levels: CtS %d, EvalMod %d, StC %d, end %d)\n",
+              Seconds(t0), req.size(), ctx->boot_param_.GetCtSStartLevel(),
+              ctx->boot_param_.GetEvalModStartLevel(), ctx->boot_param_.GetStCStartLevel(),
+              ctx->boot_param_.GetEndLevel());
+  std::fflush(stdout);
+  const EvkMap<word> &keys = ui.GetEvkMap();
+  Random rng(0x4348454444415203ull);
+
+  auto encrypt = [&](Ct &ct, const Msg &m, int level, double scale) {
+    Pt pt;
+    ctx->encoder_.Encode(pt, level, scale, m);
+    ui.Encrypt(ct, pt);
+  };
+  auto decrypt = [&](const Ct &ct) {
+    Pt pt;
+    ui.Decrypt(pt, ct);
+    Msg m;
+    ctx->encoder_.Decode(m, pt);
+    return m;
+  };
+
+  try {
+    // EvalMod: t = (I + r) / K -> ~ sin(2 pi r) / (2 pi)
+    const BootParameter &bp = ctx->boot_param_;
+    const int K = (1 << bp.num_double_angle_) * bp.initial_K_;
+    const EvalMod<word> &em = ctx->GetEvalMod();
+    Msg in(slots), plain(slots), ideal(slots);
+    for (int i = 0; i < slots; i++) {
+      const double I = (double)((int)rng.UniformInt(0, 20) - 10);
+      const double r = 0.01 * (2 * rng.UniformReal() - 1);
+      in[i] = (I + r) / K;
+      double y = em.Poly().PlainEvaluate(in[i].real() - 0.25 / K);
+      double c = std::pow(0.5 / M_PI, 1.0 / (1 << bp.num_double_angle_));
+      for (int d = 0; d < bp.num_double_angle_; d++) {
+        c *= c;
+        y = 2 * y * y - c;
+      }
+      plain[i] = y;
+      ideal[i] = std::sin(2 * M_PI * r) / (2 * M_PI);
+    }
+    Ct ct, out;
+    encrypt(ct, in, bp.GetEvalModStartLevel(), em.start_scale_);
+    ctx->EvaluateMod(out, ct, ui.GetMultiplicationKey());
+    const Msg got = decrypt(out);
+    Report("EvalMod vs plain schedule", plain, got, 1e-6);
+    Report("EvalMod vs sin(2 pi r) / (2 pi)", ideal, got, 1e-4);
+  } catch (const std::exception &e) {
+    std::printf("FAIL EvalMod: %s\n", e.what());
+    g_fail++;
+  }
+
+  try {
+    const auto &fft = ctx->GetEvalSpecialFFT(slots);
+    for (int which = 0; which < 2; which++) {
+      const auto &lt = which == 0 ? fft.CtSPhases().at(0) : fft.StCPhases().at(0);
+      Msg m(slots);
+      rng.SampleUniformComplex(m.data(), slots, -1.0, 1.0);
+      const int level = lt.PtLevel();
+      Ct ct, out;
+      encrypt(ct, m, level, P.GetRescalePrimeProd(level));
+      lt.Evaluate(*ctx, out, ct, keys);
+      Report(which == 0 ? "CoeffToSlot phase 0 vs plain" : "SlotToCoeff phase 0 vs plain",
+             lt.PlainEvaluate(m), decrypt(out), 1e-4);
+    }
+  } catch (const std::exception &e) {
+    std::printf("FAIL linear transform: %s\n", e.what());
+    g_fail++;
+  }
+
+  try {
+    Msg m(slots);
+    rng.SampleUniformComplex(m.data(), slots, -1.0, 1.0);
+    Ct ct, out;
+    encrypt(ct, m, 0, P.GetScale(0));
+    ctx->Boot(out, ct, keys);  // warm-up
+    (void)hipDeviceSynchronize();
+    double total = 0;
+    for (int r = 0; r < runs; r++) {
+      encrypt(ct, m, 0, P.GetScale(0));
+      (void)hipDeviceSynchronize();
+      auto t = Clock::now();
+      ctx->Boot(out, ct, keys);
+      (void)hipDeviceSynchronize();
+      total += Seconds(t);
+    }
+    std::printf("boot: %.2f ms per bootstrap (%d runs), output level %d\n",
+                1e3 * total / runs, runs, P.NPToLevel(out.GetNP()));
+    Report("Boot (full slots)", m, decrypt(out), 1e-3);
+  } catch (const std::exception &e) {
+    std::printf("FAIL Boot: %s\n", e.what());
+    g_fail++;
+  }
+  std::printf("OK: %d failure(s)\n", g_fail);
+  return g_fail ? 1 : 0;
+}

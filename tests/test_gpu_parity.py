@@ -207,6 +207,40 @@ def test_const_ops_and_imaginary_unit(ck, ctx24, oracle_rns24):
     assert_same(ck.to_host(o2), P.mult_imaginary_unit(y, pidx), "imaginary unit y")
 
 
+@pytest.mark.parametrize("accumulate", [False, True])
+def test_paccum_caccum(ck, ctx24, oracle_rns24, accumulate):
+    """sum over 20 terms (two launches of at most 16) of products with
+    polynomials (PAccum / giant-step accumulation) and with per-limb
+    constants (CAccum), for two polynomials at once"""
+    P = oracle_rns24
+    pidx = P.np_primes(24, 0, 8)
+    np3 = (24, 0, 8)
+    nt = 20
+    xs = [[uniform(P, pidx, SEED_BASE + 100 + 2 * t + j) for j in range(2)] for t in range(nt)]
+    ys = [uniform(P, pidx, SEED_BASE + 150 + t) for t in range(nt)]
+    rng = np.random.default_rng(SEED_BASE + 170)
+    cs = [np.array([rng.integers(0, P.prime(i)) for i in pidx], dtype=np.uint32)
+          for _ in range(nt)]
+    init = [uniform(P, pidx, SEED_BASE + 180 + j) for j in range(2)]
+    dx = [[ck.to_device(v) for v in term] for term in xs]
+    dy = [ck.to_device(v) for v in ys]
+    dc = [ck.to_device(c[None, :])[0] for c in cs]
+    for kind in ("paccum", "caccum"):
+        outs = [ck.to_device(v) for v in init]
+        if kind == "paccum":
+            ctx24.paccum(outs, dx, dy, np3, accumulate=accumulate)
+        else:
+            ctx24.caccum(outs, dx, dc, np3, accumulate=accumulate)
+        sync()
+        for j in range(2):
+            want = init[j] if accumulate else np.zeros_like(init[j])
+            for t in range(nt):
+                prod = P.mult(xs[t][j], ys[t], pidx) if kind == "paccum" else \
+                    P.mult_const(xs[t][j], cs[t], pidx)
+                want = P.add(want, prod, pidx)
+            assert_same(ck.to_host(outs[j]), want, f"{kind} poly {j} acc {accumulate}")
+
+
 @pytest.mark.parametrize("num_aux", [0, 12])
 def test_boot40_modup_to_max(ck, ctx40, oracle_boot40, num_aux):
     P = oracle_boot40
