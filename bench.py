@@ -15,6 +15,7 @@ Prints one JSON line (see the "roofline" and "cpu_baseline" objects).
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -325,6 +326,9 @@ def main():
                     help="hmult: one HMult+relin+rescale per step at L=24 (default); "
                          "batched: 64 ciphertexts at 8..24 limbs through one HMult "
                          "plan, then 16 hoisted rotations each through one plan")
+    ap.add_argument("--no-boot", action="store_true",
+                    help="skip the bootstrapping measurement (bootparam_40, full slots)")
+    ap.add_argument("--boot-runs", type=int, default=3)
     ap.add_argument("--pmc-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print per-kernel HBM traffic (JSON) from two rocprofv3 --pmc "
                          "runs (FETCH_SIZE, WRITE_SIZE) and exit")
@@ -416,6 +420,8 @@ def main():
         cpu = cpu_baseline(prm)
         log(f"cpu baseline: {cpu['value']:.3f} HMult/s")
 
+    boot = None if a.no_boot else bench_boot(a.boot_runs)
+
     res = {
         "metric": "hmult_relin_rescale_per_s",
         "value": 1e3 / ms,
@@ -437,8 +443,43 @@ def main():
                      "traffic": traffic, "launch_us": launch_us,
                      "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
+        "bootstrap": boot,
     }
     print(json.dumps(res), flush=True)
+
+
+def bench_boot(runs):
+    """Bootstrap latency and precision: cheddar-fhe_amd/boot_bench (the C++
+    facade's BootContext) on bootparam_40.json, full slots, message uniform
+    in the unit box; ms per bootstrap over `runs` after a warm-up."""
+    exe = os.path.join(REPO, "cheddar-fhe_amd", "boot_bench")
+    prm = os.path.join(REPO, "tests", "golden", "params", "bootparam_40.json")
+    if not os.path.exists(exe):
+        raise RuntimeError("cheddar-fhe_amd/boot_bench is not built (run build())")
+    log(f"bootstrap: {os.path.basename(prm)}, {runs} runs ...")
+    r = subprocess.run([exe, prm, str(runs)], capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError(f"boot_bench failed ({r.returncode}): {r.stdout[-2000:]} "
+                           f"{r.stderr[-2000:]}")
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    kern = out.pop("kernels", {})
+    total = sum(v[0] for v in kern.values())
+    by_name = {}
+    for k, (kms, calls) in kern.items():
+        base = k.split("/")[0]
+        t, c = by_name.get(base, (0.0, 0))
+        by_name[base] = (t + kms, c + calls)
+    for k, (kms, calls) in sorted(by_name.items(), key=lambda kv: -kv[1][0]):
+        log(f"  boot {k:28s} {calls:5d} launches {kms:8.3f} ms ({100 * kms / total:4.1f}%)")
+    out["kernels_by_name"] = {k: [round(v[0], 4), v[1]] for k, v in by_name.items()}
+    out["launches"] = sum(v[1] for v in by_name.values())
+    log(f"bootstrap: {out['ms_per_bootstrap']:.2f} ms, precision {out['precision_bits_max']:.2f} "
+        f"bits (max error), {out['precision_bits_mean']:.2f} bits (mean); kernel sum "
+        f"{total:.2f} ms")
+    out["kernel_ms"] = total
+    out["config"] = ("bootparam_40.json, N=2^16, 2^15 slots, dense Hamming weight = sparse "
+                     "(32), no sparse-secret encapsulation")
+    return out
 
 
 if __name__ == "__main__":

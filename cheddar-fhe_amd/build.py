@@ -56,7 +56,29 @@ def build(jobs=8, verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_boot_bench(verbose)
     return LIB
+
+
+BOOT_BENCH = os.path.join(HERE, "boot_bench")
+
+
+def build_boot_bench(verbose=False):
+    """The bootstrapping benchmark (host/boot_bench.cpp) over the C++ facade,
+    linked against the library just built."""
+    repo = os.path.dirname(HERE)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
+           "-I/opt/rocm/include", "-I" + os.path.join(repo, "include"),
+           "-I" + os.path.join(HERE, "host", "include"),
+           os.path.join(HERE, "host", "boot_bench.cpp"), "-o", BOOT_BENCH + ".tmp",
+           "-L" + HERE, "-lcheddar_amd", "-L/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"boot_bench build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(BOOT_BENCH + ".tmp", BOOT_BENCH)
+    if verbose:
+        print(f"built {BOOT_BENCH}")
 
 
 if __name__ == "__main__":

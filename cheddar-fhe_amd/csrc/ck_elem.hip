@@ -425,45 +425,48 @@ void launch_binary(const Context &ctx, int op, const EwNp &np,
                    const EwOperand &x, const EwOperand &y, const EwOperand &out,
                    int npoly, hipStream_t st) {
   if (np.num_total <= 0) return;
+  const int limbs = np.num_total * npoly;
   if (op == 0)
-    hipLaunchKernelGGL(binary_kernel<0>, ew_grid(np), dim3(kThreads), 0, st, np,
-                       ctx.ptab, x, y, out, npoly);
+    launch(ctx, "add", limbs, binary_kernel<0>, ew_grid(np), dim3(kThreads), st, np,
+           ctx.ptab, x, y, out, npoly);
   else if (op == 1)
-    hipLaunchKernelGGL(binary_kernel<1>, ew_grid(np), dim3(kThreads), 0, st, np,
-                       ctx.ptab, x, y, out, npoly);
+    launch(ctx, "sub", limbs, binary_kernel<1>, ew_grid(np), dim3(kThreads), st, np,
+           ctx.ptab, x, y, out, npoly);
   else
-    hipLaunchKernelGGL(binary_kernel<2>, ew_grid(np), dim3(kThreads), 0, st, np,
-                       ctx.ptab, x, y, out, npoly);
+    launch(ctx, "mult", limbs, binary_kernel<2>, ew_grid(np), dim3(kThreads), st, np,
+           ctx.ptab, x, y, out, npoly);
 }
 
 void launch_unary(const Context &ctx, int op, const EwNp &np,
                   const EwOperand &x, const EwOperand &out, int npoly,
                   hipStream_t st) {
   if (np.num_total <= 0) return;
+  const int limbs = np.num_total * npoly;
   if (op == 0)
-    hipLaunchKernelGGL(unary_kernel<0>, ew_grid(np), dim3(kThreads), 0, st, np,
-                       ctx.ptab, x, out, npoly);
+    launch(ctx, "neg", limbs, unary_kernel<0>, ew_grid(np), dim3(kThreads), st, np,
+           ctx.ptab, x, out, npoly);
   else
-    hipLaunchKernelGGL(unary_kernel<1>, ew_grid(np), dim3(kThreads), 0, st, np,
-                       ctx.ptab, x, out, npoly);
+    launch(ctx, "copy", limbs, unary_kernel<1>, ew_grid(np), dim3(kThreads), st, np,
+           ctx.ptab, x, out, npoly);
 }
 
 void launch_const(const Context &ctx, int op, const EwNp &np,
                   const EwOperand &x, const uint32_t *consts, int const_gap,
                   const EwOperand &out, int npoly, hipStream_t st) {
   if (np.num_total <= 0) return;
+  const int limbs = np.num_total * npoly;
   if (op == 0)
-    hipLaunchKernelGGL(const_kernel<0>, ew_grid(np), dim3(kThreads), 0, st, np,
-                       ctx.ptab, x, consts, const_gap, out, npoly);
+    launch(ctx, "mult_const", limbs, const_kernel<0>, ew_grid(np), dim3(kThreads), st,
+           np, ctx.ptab, x, consts, const_gap, out, npoly);
   else if (op == 1)
-    hipLaunchKernelGGL(const_kernel<1>, ew_grid(np), dim3(kThreads), 0, st, np,
-                       ctx.ptab, x, consts, const_gap, out, npoly);
+    launch(ctx, "add_const", limbs, const_kernel<1>, ew_grid(np), dim3(kThreads), st,
+           np, ctx.ptab, x, consts, const_gap, out, npoly);
   else if (op == 2)
-    hipLaunchKernelGGL(const_kernel<2>, ew_grid(np), dim3(kThreads), 0, st, np,
-                       ctx.ptab, x, consts, const_gap, out, npoly);
+    launch(ctx, "sub_const", limbs, const_kernel<2>, ew_grid(np), dim3(kThreads), st,
+           np, ctx.ptab, x, consts, const_gap, out, npoly);
   else
-    hipLaunchKernelGGL(const_kernel<3>, ew_grid(np), dim3(kThreads), 0, st, np,
-                       ctx.ptab, x, consts, const_gap, out, npoly);
+    launch(ctx, "const_sub", limbs, const_kernel<3>, ew_grid(np), dim3(kThreads), st,
+           np, ctx.ptab, x, consts, const_gap, out, npoly);
 }
 
 void launch_imag_unit(const Context &ctx, const EwNp &np, const EwOperand &x,

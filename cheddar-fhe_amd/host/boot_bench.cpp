@@ -1,0 +1,113 @@
+// Bootstrapping benchmark over the C++ host facade: builds a BootContext
+// for a reference-schema parameter file (full slot count, main secret of
+// the sparse Hamming weight), generates its keys, then times `runs`
+// bootstraps of a ciphertext encrypting a message uniform in the unit box
+// (after one warm-up), checks the result and prints one JSON object:
+// ms per bootstrap, precision bits (-log2 of the max and the mean error),
+// set-up time and, with profiling on for one extra bootstrap, the summed
+// GPU time per kernel.
+//
+//   boot_bench <bootparam.json> [runs] [seed]
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "cheddar/ParameterJson.h"
+#include "cheddar/UserInterface.h"
+#include "cheddar/extension/BootContext.h"
+
+using namespace cheddar;
+using word = uint32_t;
+using Clock = std::chrono::steady_clock;
+
+static double Since(Clock::time_point t) {
+  return std::chrono::duration<double>(Clock::now() - t).count();
+}
+
+int main(int argc, char **argv) {
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: %s <bootparam.json> [runs] [seed]\n", argv[0]);
+    return 2;
+  }
+  const int runs = argc > 2 ? std::atoi(argv[2]) : 3;
+  const uint64_t seed = argc > 3 ? std::strtoull(argv[3], nullptr, 0) : 0x4348454444415204ull;
+  try {
+    std::ifstream f(argv[1]);
+    if (!f) throw Error(std::string("cannot open ") + argv[1]);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    auto param = ParameterFromJsonText(ss.str());
+    int ncts = 0, nstc = 0;
+    if (!BootLevelsFromJsonText(ss.str(), &ncts, &nstc))
+      throw Error("the parameter file does not enable bootstrapping");
+    param->SetDenseHammingWeight(param->GetSparseHammingWeight());
+    const Parameter<word> &P = *param;
+    const int slots = P.degree_ / 2;
+
+    auto t0 = Clock::now();
+    auto ctx = BootContext<word>::Create(P, BootParameter(P.max_level_, ncts, nstc));
+    UserInterface<word> ui(ctx, seed);
+    ctx->PrepareEvalMod();
+    ctx->PrepareEvalSpecialFFT(slots);
+    EvkRequest req;
+    ctx->AddRequiredRotations(req, slots);
+    ui.PrepareRotationKey(req);
+    HipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    const double setup = Since(t0);
+
+    Random rng(seed + 1);
+    std::vector<Complex> msg(slots);
+    rng.SampleUniformComplex(msg.data(), slots, -1.0, 1.0);
+    Plaintext<word> pt;
+    ctx->encoder_.Encode(pt, 0, P.GetScale(0), msg);
+    Ciphertext<word> ct, out;
+    ui.Encrypt(ct, pt);
+    ctx->Boot(out, ct, ui.GetEvkMap());  // warm-up
+    HipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    double total = 0;
+    for (int r = 0; r < runs; r++) {
+      auto t = Clock::now();
+      ctx->Boot(out, ct, ui.GetEvkMap());
+      HipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+      total += Since(t);
+    }
+    Plaintext<word> dec;
+    ui.Decrypt(dec, out);
+    std::vector<Complex> got;
+    ctx->encoder_.Decode(got, dec);
+    double worst = 0, sum = 0;
+    for (int i = 0; i < slots; i++) {
+      const double e = std::abs(got[i] - msg[i]);
+      if (!(e <= worst)) worst = e;
+      sum += e;
+    }
+
+    // one more bootstrap with per-kernel timing
+    Context<word>::Check(ck_profile_enable(ctx->Handle(), 1));
+    ctx->Boot(out, ct, ui.GetEvkMap());
+    constexpr int kMax = 1024;
+    std::vector<char> names(64 * kMax);
+    std::vector<double> ms(kMax);
+    std::vector<long long> calls(kMax);
+    const int n = ck_profile_report(ctx->Handle(), kMax, names.data(), ms.data(), calls.data());
+    Context<word>::Check(ck_profile_enable(ctx->Handle(), 0));
+
+    std::printf("{\"ms_per_bootstrap\": %.3f, \"runs\": %d, \"precision_bits_max\": %.2f, "
+                "\"precision_bits_mean\": %.2f, \"max_error\": %.3e, \"setup_s\": %.1f, "
+                "\"rotation_keys\": %zu, \"slots\": %d, \"end_level\": %d, \"kernels\": {",
+                1e3 * total / runs, runs, -std::log2(worst), -std::log2(sum / slots), worst,
+                setup, req.size(), slots, P.NPToLevel(out.GetNP()));
+    for (int i = 0; i < n; i++)
+      std::printf("%s\"%s\": [%.4f, %lld]", i ? ", " : "", &names[64 * i], ms[i], calls[i]);
+    std::printf("}}\n");
+    return worst < 1e-3 ? 0 : 1;
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "boot_bench: %s\n", e.what());
+    return 1;
+  }
+}

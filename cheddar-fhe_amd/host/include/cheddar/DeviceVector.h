@@ -1,14 +1,24 @@
 // C++ host mirror of the reference's DeviceVector (include/core/
 // DeviceVector.h): an owning, move-only array of words in GPU memory.
-// Allocation uses hipMalloc; errors throw cheddar::Error (the reference
-// asserts and exits instead, include/common/Assert.h:9-21).
+// Errors throw cheddar::Error (the reference asserts and exits instead,
+// include/common/Assert.h:9-21).
+//
+// Memory comes from a caching pool (the reference allocates through an RMM
+// binning pool over stream-ordered allocation, src/core/MemoryPool.cpp):
+// blocks are kept per exact size and handed out again, so evaluation does
+// not call hipMalloc / hipFree (which synchronises the device) once the
+// working set is resident. Polynomial sizes are multiples of one limb, so
+// few distinct sizes occur. A block is reused in issue order: all work on
+// pooled memory goes to one stream (the Context's), or is synchronous.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -23,6 +33,52 @@ inline void HipCheck(hipError_t e, const char *what) {
   if (e != hipSuccess)
     throw Error(std::string(what) + ": " + hipGetErrorString(e));
 }
+
+class DevicePool {
+ public:
+  static DevicePool &Get() {
+    static DevicePool *pool = new DevicePool;  // outlives every DeviceVector
+    return *pool;
+  }
+  void *Alloc(size_t bytes) {
+    {
+      std::lock_guard<std::mutex> g(m_);
+      auto it = free_.find(bytes);
+      if (it != free_.end() && !it->second.empty()) {
+        void *p = it->second.back();
+        it->second.pop_back();
+        cached_ -= bytes;
+        return p;
+      }
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      Trim();  // give cached blocks back and try once more
+      HipCheck(hipMalloc(&p, bytes), "hipMalloc");
+    }
+    return p;
+  }
+  void Free(void *p, size_t bytes) {
+    std::lock_guard<std::mutex> g(m_);
+    free_[bytes].push_back(p);
+    cached_ += bytes;
+  }
+  // return every cached block to the driver
+  void Trim() {
+    std::lock_guard<std::mutex> g(m_);
+    for (auto &kv : free_)
+      for (void *p : kv.second) (void)hipFree(p);
+    free_.clear();
+    cached_ = 0;
+  }
+  size_t CachedBytes() const { return cached_; }
+
+ private:
+  std::mutex m_;
+  std::unordered_map<size_t, std::vector<void *>> free_;
+  size_t cached_ = 0;
+};
 
 template <typename word>
 class DeviceVector {
@@ -52,14 +108,22 @@ class DeviceVector {
   // kept.
   void resize(size_t size) {
     if (size == size_) return;
-    word *p = nullptr;
-    if (size) HipCheck(hipMalloc((void **)&p, size * sizeof(word)), "hipMalloc");
+    word *p = size ? (word *)DevicePool::Get().Alloc(size * sizeof(word)) : nullptr;
     if (ptr_ && p && size_)
       HipCheck(hipMemcpy(p, ptr_, std::min(size, size_) * sizeof(word),
                          hipMemcpyDeviceToDevice),
                "hipMemcpy");
     release();
     ptr_ = p;
+    size_ = size;
+  }
+
+  // Reallocates when the size changes, without keeping the contents (for
+  // outputs that are about to be overwritten).
+  void reset(size_t size) {
+    if (size == size_) return;
+    release();
+    ptr_ = size ? (word *)DevicePool::Get().Alloc(size * sizeof(word)) : nullptr;
     size_ = size;
   }
 
@@ -82,7 +146,7 @@ class DeviceVector {
 
  private:
   void release() {
-    if (ptr_) (void)hipFree(ptr_);
+    if (ptr_) DevicePool::Get().Free(ptr_, size_ * sizeof(word));
     ptr_ = nullptr;
     size_ = 0;
   }

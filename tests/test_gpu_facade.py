@@ -4,7 +4,8 @@ UserInterface API) driven by tests/cpp/test_context.cpp (parameter JSON in
 the reference's schema, HMult / Relinearize(Rescale) / Rescale / HRot /
 HConj / hoisted HRot / MultKeyNoModDown / Add / Sub / Neg on the GPU, each
 output compared word for word with the C oracle inside that program) and
-tests/cpp/test_basic.cpp (encrypted semantics)."""
+tests/cpp/test_basic.cpp (encrypted semantics) and tests/cpp/test_boot.cpp
+(bootstrapping)."""
 import os
 import subprocess
 
@@ -16,6 +17,7 @@ pytestmark = pytest.mark.gpu
 
 BIN = os.path.join(REPO, "tests", "cpp", "test_context")
 BASIC = os.path.join(REPO, "tests", "cpp", "test_basic")
+BOOT = os.path.join(REPO, "tests", "cpp", "test_boot")
 
 
 @pytest.mark.parametrize("params,level", [
@@ -48,5 +50,18 @@ def test_cpp_basic_semantics(params, weight):
     args = [BASIC, params] + ([str(weight)] if weight else [])
     r = subprocess.run(args, capture_output=True, text=True, timeout=600)
     print(r.stdout[-6000:])
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    assert "OK: 0 failure(s)" in r.stdout
+
+
+# Bootstrapping (unittest/Bootstrapping.cpp): bootparam_40, full slots,
+# message uniform in the unit box, max error 1e-3 as the reference's test;
+# stage checks (EvalMod, first CoeffToSlot / SlotToCoeff phases) inside.
+def test_cpp_bootstrap():
+    assert os.path.exists(BOOT), "tests/cpp/test_boot is not built (run build())"
+    print("bootstrap bootparam_40", flush=True)
+    r = subprocess.run([BOOT, os.path.join(GOLDEN, "params", "bootparam_40.json"), "1"],
+                       capture_output=True, text=True, timeout=600)
+    print(r.stdout[-4000:])
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
     assert "OK: 0 failure(s)" in r.stdout
