@@ -477,8 +477,8 @@ def bench_boot(runs):
         f"bits (max error), {out['precision_bits_mean']:.2f} bits (mean); kernel sum "
         f"{total:.2f} ms")
     out["kernel_ms"] = total
-    out["config"] = ("bootparam_40.json, N=2^16, 2^15 slots, dense Hamming weight = sparse "
-                     "(32), no sparse-secret encapsulation")
+    out["config"] = ("bootparam_40.json as shipped: N=2^16, 2^15 slots, dense secret "
+                     "(Hamming weight 32768) with sparse-secret encapsulation (32)")
     return out
 
 

@@ -62,6 +62,7 @@ _SIGS = {
                                               ctypes.POINTER(_vp), _vp]),
     "ck_modup_to_max": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int,
                                        _vp]),
+    "ck_crt_lift": (ctypes.c_int, [_vp, _ip, ctypes.c_int, _ip, ctypes.c_int, _vp, _vp, _vp]),
     "ck_paccum": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                  ctypes.POINTER(_vp), ctypes.c_int, _vp]),
@@ -304,6 +305,12 @@ class Context:
         flat = [v for term in xs for v in term]
         _check(lib.ck_caccum(self._h, *np3, len(dsts), len(consts), _ptrs(dsts), _ptrs(flat),
                              _ptrs(consts), int(accumulate), _stream(stream)))
+
+    def crt_lift(self, dst, src, src_primes, dst_primes, stream=None):
+        sp = (ctypes.c_int * len(src_primes))(*src_primes)
+        dp = (ctypes.c_int * len(dst_primes))(*dst_primes)
+        _check(lib.ck_crt_lift(self._h, sp, len(src_primes), dp, len(dst_primes), _ptr(src),
+                               _ptr(dst), _stream(stream)))
 
     def tensor(self, d0, d1, d2, b1, a1, b2, a2, num_main, num_ter, stream=None):
         _check(lib.ck_tensor(self._h, num_main, num_ter, _ptr(d0), _ptr(d1), _ptr(d2),

@@ -325,10 +325,39 @@ int ck_modup_to_max(ck_context *ctx, int base_main, int base_ter,
   CK_REQUIRE(num_aux >= 0 && num_aux <= c.alpha, "invalid num_aux");
   CK_REQUIRE(dst != src, "dst must not alias src");
   const int p0 = c.max_ter - base_ter;
-  const uint32_t q0 = c.primes[p0], q1 = nbase == 2 ? c.primes[p0 + 1] : 0u;
-  EwNp np = make_np(c, c.max_main, c.max_ter, num_aux);
-  launch_modup_to_max(c, np, src, nbase, q0, q1, dst, as_stream(stream));
+  LiftArgs A{};
+  A.nsrc = nbase;
+  A.q0 = c.primes[p0];
+  A.q1 = nbase == 2 ? c.primes[p0 + 1] : 0u;
+  const int ndst = c.L + num_aux;  // every q prime (global order), then aux
+  CK_REQUIRE(ndst <= kMaxLiftDst, "too many destination limbs");
+  for (int y = 0; y < ndst; y++) A.dst_prime[y] = (uint16_t)y;
+  launch_crt_lift(c, src, A, ndst, dst, as_stream(stream));
   return check_hip(hipGetLastError(), "modup_to_max");
+}
+
+int ck_crt_lift(ck_context *ctx, const int *src_primes, int n_src,
+                const int *dst_primes, int n_dst, const uint32_t *src, uint32_t *dst,
+                void *stream) {
+  CK_REQUIRE(ctx && ctx->c && src_primes && dst_primes && src && dst, "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(n_src == 1 || n_src == 2, "one or two source primes");
+  CK_REQUIRE(n_dst >= 1 && n_dst <= kMaxLiftDst, "1..128 destination primes");
+  CK_REQUIRE(dst != src, "dst must not alias src");
+  const int np = (int)c.primes.size();
+  for (int i = 0; i < n_src; i++)
+    CK_REQUIRE(src_primes[i] >= 0 && src_primes[i] < np, "invalid source prime");
+  CK_REQUIRE(n_src == 1 || src_primes[0] != src_primes[1], "source primes must differ");
+  LiftArgs A{};
+  A.nsrc = n_src;
+  A.q0 = c.primes[src_primes[0]];
+  A.q1 = n_src == 2 ? c.primes[src_primes[1]] : 0u;
+  for (int y = 0; y < n_dst; y++) {
+    CK_REQUIRE(dst_primes[y] >= 0 && dst_primes[y] < np, "invalid destination prime");
+    A.dst_prime[y] = (uint16_t)dst_primes[y];
+  }
+  launch_crt_lift(c, src, A, n_dst, dst, as_stream(stream));
+  return check_hip(hipGetLastError(), "crt_lift");
 }
 
 static int accum_common(ck_context *ctx, int num_main, int num_ter, int num_aux,

@@ -119,9 +119,16 @@ void launch_const(const Context &ctx, int op, const EwNp &np,
                   const EwOperand &out, int npoly, hipStream_t st);
 void launch_imag_unit(const Context &ctx, const EwNp &np, const EwOperand &x,
                       const EwOperand &out, int npoly, hipStream_t st);
-void launch_modup_to_max(const Context &ctx, const EwNp &np, const uint32_t *src,
-                         int nbase, uint32_t q0, uint32_t q1, uint32_t *dst,
-                         hipStream_t st);
+// Exact CRT lift of one- or two-prime coefficient-domain residues to up to
+// kMaxLiftDst destination primes (global prime indices).
+constexpr int kMaxLiftDst = 128;
+struct LiftArgs {
+  int nsrc;
+  uint32_t q0, q1;
+  uint16_t dst_prime[kMaxLiftDst];
+};
+void launch_crt_lift(const Context &ctx, const uint32_t *src, const LiftArgs &A,
+                     int ndst, uint32_t *dst, hipStream_t st);
 void launch_accum(const Context &ctx, const EwNp &np, const AccumArgs &A,
                   hipStream_t st);
 void launch_tensor(const Context &ctx, const EwNp &np, const EwOperand &c1,

@@ -146,21 +146,26 @@ __global__ __launch_bounds__(kThreads) void accum_kernel(const EwNp np,
   }
 }
 
-// ModUpToMax1 / ModUpToMax2 (ElementWise.cu:579-623): coefficient-domain
-// limbs of a one- or two-prime base lifted to every destination prime.
-// Two primes: v = s0 q1 + s1 q0 in [0, 2 q0 q1), centred into
-// (-q0 q1 / 2, q0 q1 / 2], then reduced; one prime: s0 mod p (no centring).
-__global__ __launch_bounds__(kThreads) void modup_to_max_kernel(
-    const EwNp np, const PrimeTab P, const uint32_t *src, int nbase,
-    uint32_t q0, uint32_t q1, uint32_t *dst) {
-  EW_PROLOGUE
+// ModUpToMax1 / ModUpToMax2 (ElementWise.cu:579-623), generalised to any
+// source and destination primes: coefficient-domain limbs of a one- or
+// two-prime base lifted to every destination prime. Two primes: v = s0 q1
+// + s1 q0 in [0, 2 q0 q1), centred into (-q0 q1 / 2, q0 q1 / 2], then
+// reduced; one prime: s0 mod p (not centred). With s_i = [x (q0 q1 /
+// q_i)^-1]_{q_i} this is the exact CRT lift of x.
+__global__ __launch_bounds__(kThreads) void crt_lift_kernel(const PrimeTab P,
+                                                            const uint32_t *src,
+                                                            const LiftArgs A,
+                                                            uint32_t *dst) {
+  const int y = blockIdx.y;
+  const int k = (blockIdx.x * kThreads + threadIdx.x) * kVec;
+  const uint32_t q = P.q[A.dst_prime[y]];
   const uint4 s0 = ld4(src + k);
   uint4 r;
-  if (nbase == 2) {
+  if (A.nsrc == 2) {
     const uint4 s1 = ld4(src + 65536 + k);
-    const int64_t qq = (int64_t)q0 * q1, half = qq >> 1;
+    const int64_t qq = (int64_t)A.q0 * A.q1, half = qq >> 1;
     auto lift = [&](uint32_t u, uint32_t v) {
-      int64_t t = (int64_t)u * q1 + (int64_t)v * q0;
+      int64_t t = (int64_t)u * A.q1 + (int64_t)v * A.q0;
       if (t >= qq) t -= qq;
       if (t > half) t -= qq;
       int64_t m = t % (int64_t)q;
@@ -484,12 +489,11 @@ void launch_accum(const Context &ctx, const EwNp &np, const AccumArgs &A,
          dim3(kThreads), st, np, ctx.ptab, A);
 }
 
-void launch_modup_to_max(const Context &ctx, const EwNp &np, const uint32_t *src,
-                         int nbase, uint32_t q0, uint32_t q1, uint32_t *dst,
-                         hipStream_t st) {
-  if (np.num_total <= 0) return;
-  launch(ctx, "modup_to_max", np.num_total, modup_to_max_kernel, ew_grid(np),
-         dim3(kThreads), st, np, ctx.ptab, src, nbase, q0, q1, dst);
+void launch_crt_lift(const Context &ctx, const uint32_t *src, const LiftArgs &A,
+                     int ndst, uint32_t *dst, hipStream_t st) {
+  if (ndst <= 0) return;
+  launch(ctx, "crt_lift", ndst, crt_lift_kernel, dim3(kBlocksPerLimb, ndst),
+         dim3(kThreads), st, ctx.ptab, src, A, dst);
 }
 
 void launch_tensor(const Context &ctx, const EwNp &np, const EwOperand &c1,

@@ -7,7 +7,10 @@
 // set-up time and, with profiling on for one extra bootstrap, the summed
 // GPU time per kernel.
 //
-//   boot_bench <bootparam.json> [runs] [seed]
+//   boot_bench <bootparam.json> [runs] [seed] [sse|nosse]
+//
+// sse (default): the file's dense Hamming weight with sparse-secret
+// encapsulation; nosse: the main secret gets the sparse weight.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -36,6 +39,7 @@ int main(int argc, char **argv) {
   }
   const int runs = argc > 2 ? std::atoi(argv[2]) : 3;
   const uint64_t seed = argc > 3 ? std::strtoull(argv[3], nullptr, 0) : 0x4348454444415204ull;
+  const bool sse = !(argc > 4 && std::string(argv[4]) == "nosse");
   try {
     std::ifstream f(argv[1]);
     if (!f) throw Error(std::string("cannot open ") + argv[1]);
@@ -45,7 +49,7 @@ int main(int argc, char **argv) {
     int ncts = 0, nstc = 0;
     if (!BootLevelsFromJsonText(ss.str(), &ncts, &nstc))
       throw Error("the parameter file does not enable bootstrapping");
-    param->SetDenseHammingWeight(param->GetSparseHammingWeight());
+    if (!sse) param->SetDenseHammingWeight(param->GetSparseHammingWeight());
     const Parameter<word> &P = *param;
     const int slots = P.degree_ / 2;
 
@@ -99,9 +103,13 @@ int main(int argc, char **argv) {
 
     std::printf("{\"ms_per_bootstrap\": %.3f, \"runs\": %d, \"precision_bits_max\": %.2f, "
                 "\"precision_bits_mean\": %.2f, \"max_error\": %.3e, \"setup_s\": %.1f, "
-                "\"rotation_keys\": %zu, \"slots\": %d, \"end_level\": %d, \"kernels\": {",
+                "\"rotation_keys\": %zu, \"slots\": %d, \"end_level\": %d, "
+                "\"dense_hamming_weight\": %d, \"sparse_hamming_weight\": %d, "
+                "\"sparse_secret_encapsulation\": %s, \"kernels\": {",
                 1e3 * total / runs, runs, -std::log2(worst), -std::log2(sum / slots), worst,
-                setup, req.size(), slots, P.NPToLevel(out.GetNP()));
+                setup, req.size(), slots, P.NPToLevel(out.GetNP()),
+                P.GetDenseHammingWeight(), P.GetSparseHammingWeight(),
+                P.IsUsingSparseSecretEncapsulation() ? "true" : "false");
     for (int i = 0; i < n; i++)
       std::printf("%s\"%s\": [%.4f, %lld]", i ? ", " : "", &names[64 * i], ms[i], calls[i]);
     std::printf("}}\n");

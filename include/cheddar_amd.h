@@ -167,6 +167,15 @@ int ck_mult_imaginary_unit(ck_context *ctx, int num_main, int num_ter,
 int ck_modup_to_max(ck_context *ctx, int base_main, int base_ter,
                     const uint32_t *src, uint32_t *dst, int num_aux,
                     void *stream);
+/* The same lift between explicit global prime indices (the reference's
+ * ModUpToMax kernels with free source / destination primes; used for the
+ * two-prime key switches of sparse-secret encapsulation,
+ * src/extension/BootContext.cpp:193-268): src = n_src (1 or 2)
+ * coefficient-domain limbs s_i = [x (Q / q_i)^-1]_{q_i}; dst = x mod p for
+ * each of n_dst (<= 128) destination primes, x centred for two primes. */
+int ck_crt_lift(ck_context *ctx, const int *src_primes, int n_src,
+                const int *dst_primes, int n_dst, const uint32_t *src, uint32_t *dst,
+                void *stream);
 /* ElementWiseHandler::PAccum (ElementWise.cu:1437-1643, CPAccum kernels
  * :225-366) and the giant-step accumulation GSFusedKernel
  * (src/extension/Hoist.cu:80-118): for p < npoly (1 or 2),

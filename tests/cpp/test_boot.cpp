@@ -8,10 +8,11 @@
 // - the full bootstrap: precision bits = -log2(max error) and the mean
 //   error, and the wall time per bootstrap (after one warm-up).
 //
-//   test_boot <bootparam.json> [timed_runs]
+//   test_boot <bootparam.json> [timed_runs] [sse]
 //
-// The main secret has Hamming weight equal to the sparse weight of the
-// parameter file (no sparse-secret encapsulation).
+// By default the main secret has the sparse Hamming weight of the parameter
+// file (no sparse-secret encapsulation); with "sse" the file's dense weight
+// is kept and ModUpToMax switches to the sparse secret and back.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -62,6 +63,7 @@ int main(int argc, char **argv) {
     return 2;
   }
   const int runs = argc > 2 ? std::atoi(argv[2]) : 2;
+  const bool sse = argc > 3 && std::string(argv[3]) == "sse";
   std::ifstream f(argv[1]);
   std::stringstream ss;
   ss << f.rdbuf();
@@ -72,9 +74,12 @@ int main(int argc, char **argv) {
     std::fprintf(stderr, "%s does not enable bootstrapping\n", argv[1]);
     return 2;
   }
-  param->SetDenseHammingWeight(param->GetSparseHammingWeight());
+  if (!sse) param->SetDenseHammingWeight(param->GetSparseHammingWeight());
   const Parameter<word> &P = *param;
   const int slots = P.degree_ / 2;
+  std::printf("secret: dense Hamming weight %d, sparse %d, encapsulation %s\n",
+              P.GetDenseHammingWeight(), P.GetSparseHammingWeight(),
+              P.IsUsingSparseSecretEncapsulation() ? "on" : "off");
 
   auto t0 = Clock::now();
   auto ctx = BootContext<word>::Create(P, BootParameter(P.max_level_, ncts, nstc));

@@ -57,11 +57,16 @@ def test_cpp_basic_semantics(params, weight):
 # Bootstrapping (unittest/Bootstrapping.cpp): bootparam_40, full slots,
 # message uniform in the unit box, max error 1e-3 as the reference's test;
 # stage checks (EvalMod, first CoeffToSlot / SlotToCoeff phases) inside.
-def test_cpp_bootstrap():
+# "sse": the file's dense secret (Hamming weight 32768) with sparse-secret
+# encapsulation; "dense=sparse": a weight-32 main secret, no encapsulation.
+@pytest.mark.parametrize("mode", ["sse", "dense=sparse"])
+def test_cpp_bootstrap(mode):
     assert os.path.exists(BOOT), "tests/cpp/test_boot is not built (run build())"
-    print("bootstrap bootparam_40", flush=True)
-    r = subprocess.run([BOOT, os.path.join(GOLDEN, "params", "bootparam_40.json"), "1"],
-                       capture_output=True, text=True, timeout=600)
+    print(f"bootstrap bootparam_40 {mode}", flush=True)
+    args = [BOOT, os.path.join(GOLDEN, "params", "bootparam_40.json"), "1"]
+    if mode == "sse":
+        args.append("sse")
+    r = subprocess.run(args, capture_output=True, text=True, timeout=600)
     print(r.stdout[-4000:])
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
     assert "OK: 0 failure(s)" in r.stdout

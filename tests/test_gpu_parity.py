@@ -254,6 +254,22 @@ def test_boot40_modup_to_max(ck, ctx40, oracle_boot40, num_aux):
     assert_same(ck.to_host(out), want, f"modup_to_max aux {num_aux}")
 
 
+def test_boot40_crt_lift(ck, ctx40, oracle_boot40):
+    """exact lift between explicit primes both ways (the two-prime key
+    switches of sparse-secret encapsulation): short base -> first two
+    auxiliary primes, and back, and one prime -> many"""
+    P = oracle_boot40
+    short = P.np_primes(0, 2)
+    aux = [P.num_primes - P.alpha, P.num_primes - P.alpha + 1]
+    for src_p, dst_p, seed in ((short, aux, 92), (aux, short, 93), ([aux[0]], list(range(20)), 94)):
+        src = uniform(P, src_p, SEED_BASE + seed)
+        want = P.modup_to_max(src, src_p, dst_p)
+        out = torch.empty((len(dst_p), P.n), dtype=torch.int32, device="cuda")
+        ctx40.crt_lift(out, ck.to_device(src), src_p, dst_p)
+        sync()
+        assert_same(ck.to_host(out), want, f"crt_lift {src_p} -> {dst_p[:4]}")
+
+
 def test_modup_to_max_single_prime(ck, ctx24, oracle_rns24):
     P = oracle_rns24
     base = P.np_primes(1, 0)
