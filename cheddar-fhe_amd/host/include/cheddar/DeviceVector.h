@@ -36,13 +36,47 @@ inline void HipCheck(hipError_t e, const char *what) {
 
 class DevicePool {
  public:
+  // Blocks freed while a scope is active stay with the scope (and are handed
+  // out again only inside it) until the scope is released: a captured HIP
+  // graph keeps using the temporaries it was recorded with, so they must
+  // not go to other owners while the graph lives.
+  struct Scope {
+    std::unordered_map<size_t, std::vector<void *>> free;
+  };
+
   static DevicePool &Get() {
     static DevicePool *pool = new DevicePool;  // outlives every DeviceVector
     return *pool;
   }
+  void BeginScope(Scope *s) {
+    std::lock_guard<std::mutex> g(m_);
+    scope_ = s;
+  }
+  void EndScope() {
+    std::lock_guard<std::mutex> g(m_);
+    scope_ = nullptr;
+  }
+  // give a finished scope's blocks back to the pool
+  void ReleaseScope(Scope *s) {
+    std::lock_guard<std::mutex> g(m_);
+    for (auto &kv : s->free)
+      for (void *p : kv.second) {
+        free_[kv.first].push_back(p);
+        cached_ += kv.first;
+      }
+    s->free.clear();
+  }
   void *Alloc(size_t bytes) {
     {
       std::lock_guard<std::mutex> g(m_);
+      if (scope_) {
+        auto it = scope_->free.find(bytes);
+        if (it != scope_->free.end() && !it->second.empty()) {
+          void *p = it->second.back();
+          it->second.pop_back();
+          return p;
+        }
+      }
       auto it = free_.find(bytes);
       if (it != free_.end() && !it->second.empty()) {
         void *p = it->second.back();
@@ -61,6 +95,10 @@ class DevicePool {
   }
   void Free(void *p, size_t bytes) {
     std::lock_guard<std::mutex> g(m_);
+    if (scope_) {
+      scope_->free[bytes].push_back(p);
+      return;
+    }
     free_[bytes].push_back(p);
     cached_ += bytes;
   }
@@ -78,6 +116,7 @@ class DevicePool {
   std::mutex m_;
   std::unordered_map<size_t, std::vector<void *>> free_;
   size_t cached_ = 0;
+  Scope *scope_ = nullptr;
 };
 
 template <typename word>

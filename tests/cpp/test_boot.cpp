@@ -6,7 +6,10 @@
 // - the first CoeffToSlot and SlotToCoeff phases against the plain
 //   baby-step / giant-step product of their plaintext maps;
 // - the full bootstrap: precision bits = -log2(max error) and the mean
-//   error, and the wall time per bootstrap (after one warm-up).
+//   error, and the wall time per bootstrap (after one warm-up);
+// - four bootstraps captured into one HIP graph: three replays, with eager
+//   work and synchronisation between them, each output equal word for word
+//   to the eager bootstrap of the same input.
 //
 //   test_boot <bootparam.json> [timed_runs] [sse]
 //
@@ -23,6 +26,7 @@
 #include "cheddar/ParameterJson.h"
 #include "cheddar/UserInterface.h"
 #include "cheddar/extension/BootContext.h"
+#include "cheddar/extension/BootGraph.h"
 
 using namespace cheddar;
 using word = uint32_t;
@@ -180,6 +184,54 @@ int main(int argc, char **argv) {
     Report("Boot (full slots)", m, decrypt(out), 1e-3);
   } catch (const std::exception &e) {
     std::printf("FAIL Boot: %s\n", e.what());
+    g_fail++;
+  }
+  try {
+    const int batch = 4;
+    std::vector<Ct> in(batch), out(batch), eager(batch);
+    std::vector<Msg> msgs(batch, Msg(slots));
+    for (int i = 0; i < batch; i++) {
+      rng.SampleUniformComplex(msgs[i].data(), slots, -1.0, 1.0);
+      encrypt(in[i], msgs[i], 0, P.GetScale(0));
+      ctx->Boot(eager[i], in[i], keys);
+    }
+    hipStream_t st;
+    HipCheck(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+    {
+      std::vector<const Ct *> ip;
+      std::vector<Ct *> op;
+      for (int i = 0; i < batch; i++) {
+        ip.push_back(&in[i]);
+        op.push_back(&out[i]);
+      }
+      HipCheck(hipDeviceSynchronize(), "sync");
+      BootGraph<word> graph(*ctx, ip, op, keys, st);
+      std::printf("graph: %d bootstraps, %zu nodes\n", batch, graph.Nodes());
+      for (int rep = 0; rep < 3; rep++) {
+        Ct scratch;
+        ctx->Boot(scratch, in[0], keys);  // eager work between replays
+        HipCheck(hipDeviceSynchronize(), "sync");
+        graph.Launch();
+        HipCheck(hipStreamSynchronize(st), "sync");
+        bool same = true;
+        for (int i = 0; i < batch; i++)
+          same = same && out[i].bx_.CopyToHost() == eager[i].bx_.CopyToHost() &&
+                 out[i].ax_.CopyToHost() == eager[i].ax_.CopyToHost();
+        std::printf("%s graph replay %d equals the eager bootstraps\n", same ? "PASS" : "FAIL",
+                    rep);
+        if (!same) g_fail++;
+      }
+      auto t = Clock::now();
+      graph.Launch();
+      HipCheck(hipStreamSynchronize(st), "sync");
+      std::printf("graph: %.2f ms per bootstrap (%d in one graph)\n",
+                  1e3 * Seconds(t) / batch, batch);
+      for (int i = 0; i < batch; i++)
+        Report("graph bootstrap " + std::to_string(i), msgs[i], decrypt(out[i]), 1e-3);
+    }
+    HipCheck(hipStreamDestroy(st), "hipStreamDestroy");
+  } catch (const std::exception &e) {
+    std::printf("FAIL BootGraph: %s\n", e.what());
     g_fail++;
   }
   std::printf("OK: %d failure(s)\n", g_fail);
