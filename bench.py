@@ -475,7 +475,9 @@ def bench_boot(runs):
     out["launches"] = sum(v[1] for v in by_name.values())
     log(f"bootstrap: {out['ms_per_bootstrap']:.2f} ms, precision {out['precision_bits_max']:.2f} "
         f"bits (max error), {out['precision_bits_mean']:.2f} bits (mean); kernel sum "
-        f"{total:.2f} ms")
+        f"{total:.2f} ms; {out['graph_batch']} in one captured graph: "
+        f"{out['graph_ms_per_bootstrap']:.2f} ms per bootstrap "
+        f"({out['graph_precision_bits_max']:.2f} bits)")
     out["kernel_ms"] = total
     out["config"] = ("bootparam_40.json as shipped: N=2^16, 2^15 slots, dense secret "
                      "(Hamming weight 32768) with sparse-secret encapsulation (32)")
