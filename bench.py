@@ -242,6 +242,7 @@ def bench_batched(a, torch, ck):
     prm = ck.load_param_json(PARAMS)
     ctx = ck.Context.from_json(prm)
     ctx.set_bconv_mfma(None if a.bconv == "auto" else a.bconv == "mfma")
+    ctx.set_plan_autotune(not a.no_autotune)
     torch.cuda.set_device(0)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(0x4348454444415202)
@@ -300,7 +301,9 @@ def bench_batched(a, torch, ck):
         "config": {"workload": f"batched: {BATCH} ciphertexts at 8..24 limbs, HMult+relin+"
                                f"rescale plan then {BATCH_ROTS} hoisted HRot each",
                    "levels": levels, "rotations": rots, "log_n": 16, "alpha": ctx.alpha,
-                   "dnum": ctx.dnum, "parallelism": "single GPU"},
+                   "dnum": ctx.dnum, "parallelism": "single GPU",
+                   "autotune": not a.no_autotune,
+                   "hmult_plan": p_mult.info(), "hrot_plan": p_rot.info()},
         "hrot_per_s": hrot_s, "ms_hmult_batch": t_mult, "ms_hrot_batch": t_rot,
     }
     print(json.dumps(res), flush=True)
@@ -326,6 +329,9 @@ def main():
                     help="hmult: one HMult+relin+rescale per step at L=24 (default); "
                          "batched: 64 ciphertexts at 8..24 limbs through one HMult "
                          "plan, then 16 hoisted rotations each through one plan")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched workload: use --bconv instead of timing both BaseConv "
+                         "forms per plan")
     ap.add_argument("--no-boot", action="store_true",
                     help="skip the bootstrapping measurement (bootparam_40, full slots)")
     ap.add_argument("--boot-runs", type=int, default=3)

@@ -17,6 +17,7 @@ PARAM_DIR = os.path.join(HERE, "params")
 
 CK_MODDOWN, CK_RESCALE, CK_MODDOWN_RESCALE = 0, 1, 2
 CK_OPT_BCONV_MFMA = 1
+CK_OPT_PLAN_AUTOTUNE = 2
 
 _vp = ctypes.c_void_p
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -365,6 +366,10 @@ class Context:
                                    _ptr(ct[0]), _ptr(ct[1]), _ptrs(kb), _ptrs(ka),
                                    int(moddown), _stream(stream)))
 
+    def set_plan_autotune(self, on=True):
+        """Batched plans built in both BaseConv forms, timed, faster kept."""
+        _check(lib.ck_context_set_option(self._h, CK_OPT_PLAN_AUTOTUNE, int(bool(on))))
+
     def set_bconv_mfma(self, on=True):
         """BaseConv on the FP64 matrix cores (True), fused into the forward
         NTT column pass (False), or None for the default (matrix cores in
@@ -432,10 +437,11 @@ class Plan:
         _check(lib.ck_plan_run(self._h, _stream(stream)))
 
     def info(self):
-        out = (ctypes.c_longlong * 4)()
+        out = (ctypes.c_longlong * 8)()
         _check(lib.ck_plan_info(self._h, out))
         return {"items": out[0], "steps": out[1], "launches": out[2],
-                "workspace_bytes": out[3]}
+                "workspace_bytes": out[3], "bconv_mfma": bool(out[4]),
+                "tuned_us": (out[5] / 1e3, out[6] / 1e3) if out[5] != -1000 else None}
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -584,6 +584,31 @@ int ck_hrot_hoisted(ck_context *ctx, int level, int n_rot, const int *rots,
                               key_a, moddown != 0, kMaxBsRot, c.ws, sink);
 }
 
+// Builds a plan with the context's BaseConv form, or with autotuning both
+// forms (fused into the forward column pass / on the matrix cores), times
+// each and keeps the faster.
+extern "C++" template <typename Build>
+static int build_tuned(const Context &c, Build &build, Plan **out) {
+  *out = nullptr;
+  if (!c.plan_autotune) return build(c.bconv_mfma != 0, out);
+  Plan *cand[2] = {nullptr, nullptr};
+  double us[2] = {-1.0, -1.0};
+  for (int m = 0; m < 2; m++) {
+    int rc = build(m == 1, &cand[m]);
+    if (rc) {
+      plan_destroy(cand[0]);
+      return rc;
+    }
+    us[m] = plan_time_us(*cand[m], 3);
+  }
+  const int best = (us[1] >= 0 && (us[0] < 0 || us[1] < us[0])) ? 1 : 0;
+  plan_destroy(cand[1 - best]);
+  cand[best]->tuned_us[0] = us[0];
+  cand[best]->tuned_us[1] = us[1];
+  *out = cand[best];
+  return kOk;
+}
+
 int ck_plan_hmult(ck_context *ctx, int n, const int *levels, int rescale,
                   uint32_t *const *out_b, uint32_t *const *out_a,
                   const uint32_t *const *b1, const uint32_t *const *a1,
@@ -609,22 +634,30 @@ int ck_plan_hmult(ck_context *ctx, int n, const int *levels, int rescale,
     off[i] = total;
     total += relin_ws_limbs(c, levels[i]);
   }
-  Plan *p = new Plan;
-  p->ctx = &c;
-  p->items = n;
-  uint32_t *ws = nullptr;
-  int rc = plan_alloc_workspace(*p, total, &ws);
-  PlanSink sink(*p);
-  for (int i = 0; i < n && !rc; i++) {
-    sink.begin_item();
-    rc = op_hmult(c, levels[i], out_b[i], out_a[i], b1[i], a1[i], b2[i], a2[i],
-                  key_b, key_a, rescale != 0, ws + off[i] * kLimb, sink);
-  }
-  if (!rc) rc = plan_finish(*p, sink.mismatch());
-  if (rc) {
-    plan_destroy(p);
+  auto build = [&](bool mfma, Plan **res) {
+    Plan *p = new Plan;
+    p->ctx = &c;
+    p->items = n;
+    p->bconv_mfma = mfma;
+    uint32_t *ws = nullptr;
+    int rc = plan_alloc_workspace(*p, total, &ws);
+    PlanSink sink(*p);
+    for (int i = 0; i < n && !rc; i++) {
+      sink.begin_item();
+      rc = op_hmult(c, levels[i], out_b[i], out_a[i], b1[i], a1[i], b2[i], a2[i],
+                    key_b, key_a, rescale != 0, ws + off[i] * kLimb, sink);
+    }
+    if (!rc) rc = plan_finish(*p, sink.mismatch());
+    if (rc) {
+      plan_destroy(p);
+      p = nullptr;
+    }
+    *res = p;
     return rc;
-  }
+  };
+  Plan *p = nullptr;
+  const int rc = build_tuned(c, build, &p);
+  if (rc) return rc;
   *out = new ck_plan{p};
   return CK_OK;
 }
@@ -657,23 +690,31 @@ int ck_plan_hrot_hoisted(ck_context *ctx, int n, const int *levels,
   for (int r = 0; r < n_rot; r++)
     CK_REQUIRE(rots[r] >= 1 && rots[r] < c.n / 2,
                "hoisted rotation amounts must be in [1, N/2)");
-  Plan *p = new Plan;
-  p->ctx = &c;
-  p->items = n;
-  uint32_t *ws = nullptr;
-  int rc = plan_alloc_workspace(*p, total, &ws);
-  PlanSink sink(*p);
-  for (int i = 0; i < n && !rc; i++) {
-    sink.begin_item();
-    rc = op_hoisted_rotations(c, levels[i], n_rot, rots, out_b + (size_t)i * n_rot,
-                              out_a + (size_t)i * n_rot, b[i], a[i], key_b, key_a,
-                              moddown != 0, n_rot, ws + off[i] * kLimb, sink);
-  }
-  if (!rc) rc = plan_finish(*p, sink.mismatch());
-  if (rc) {
-    plan_destroy(p);
+  auto build = [&](bool mfma, Plan **res) {
+    Plan *p = new Plan;
+    p->ctx = &c;
+    p->items = n;
+    p->bconv_mfma = mfma;
+    uint32_t *ws = nullptr;
+    int rc = plan_alloc_workspace(*p, total, &ws);
+    PlanSink sink(*p);
+    for (int i = 0; i < n && !rc; i++) {
+      sink.begin_item();
+      rc = op_hoisted_rotations(c, levels[i], n_rot, rots, out_b + (size_t)i * n_rot,
+                                out_a + (size_t)i * n_rot, b[i], a[i], key_b, key_a,
+                                moddown != 0, n_rot, ws + off[i] * kLimb, sink);
+    }
+    if (!rc) rc = plan_finish(*p, sink.mismatch());
+    if (rc) {
+      plan_destroy(p);
+      p = nullptr;
+    }
+    *res = p;
     return rc;
-  }
+  };
+  Plan *p = nullptr;
+  const int rc = build_tuned(c, build, &p);
+  if (rc) return rc;
   *out = new ck_plan{p};
   return CK_OK;
 }
@@ -689,6 +730,10 @@ int ck_plan_info(const ck_plan *plan, long long *out) {
   out[1] = (long long)plan->p->stages.size();
   out[2] = plan_launches(*plan->p);
   out[3] = (long long)plan->p->workspace_bytes;
+  out[4] = plan->p->bconv_mfma ? 1 : 0;
+  out[5] = (long long)(plan->p->tuned_us[0] * 1000.0);  // ns, -1000: not tuned
+  out[6] = (long long)(plan->p->tuned_us[1] * 1000.0);
+  out[7] = 0;
   return CK_OK;
 }
 
@@ -705,6 +750,10 @@ int ck_context_set_option(ck_context *ctx, int option, int value) {
     case CK_OPT_BCONV_MFMA:
       CK_REQUIRE(value >= -1 && value <= 1, "option value must be -1, 0 or 1");
       ctx->c->bconv_mfma = value;
+      return CK_OK;
+    case CK_OPT_PLAN_AUTOTUNE:
+      CK_REQUIRE(value == 0 || value == 1, "option value must be 0 or 1");
+      ctx->c->plan_autotune = value != 0;
       return CK_OK;
     default:
       return fail(CK_ERR_INVALID_ARG, "unknown context option");

@@ -157,6 +157,8 @@ struct Context {
   // the forward column pass: 1 always, 0 never, -1 (default) for batched
   // plans only, where it measured faster
   int bconv_mfma = -1;
+  // batched plans: build both BaseConv forms, time them, keep the faster
+  bool plan_autotune = false;
 
   // workspace for multi-kernel operations (one operation at a time per
   // context; concurrent calls on different streams race)

@@ -198,6 +198,31 @@ void plan_destroy(Plan *p) {
   delete p;
 }
 
+double plan_time_us(const Plan &p, int reps) {
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double us = -1.0;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
+      hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
+      hipDeviceSynchronize() == hipSuccess && plan_run(p, st) == kOk &&
+      hipEventRecord(e0, st) == hipSuccess) {
+    bool ok = true;
+    for (int r = 0; r < reps && ok; r++) ok = plan_run(p, st) == kOk;
+    float ms = 0.f;
+    if (ok && hipEventRecord(e1, st) == hipSuccess &&
+        hipEventSynchronize(e1) == hipSuccess &&
+        hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+      us = 1e3 * ms / reps;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) {
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+  }
+  return us;
+}
+
 int plan_launches(const Plan &p) {
   int n = 0;
   for (const PlanStage &s : p.stages)

@@ -28,6 +28,10 @@ struct Plan {
   std::vector<void *> allocations;
   size_t workspace_bytes = 0;
   int items = 0;
+  bool bconv_mfma = true;  // BaseConv steps on the matrix cores
+  // autotuning record: measured run time of each BaseConv form (us; < 0:
+  // not measured)
+  double tuned_us[2] = {-1.0, -1.0};
 };
 
 // Records the ops of one item after another; begin_item() rewinds to the
@@ -35,7 +39,7 @@ struct Plan {
 class PlanSink : public Sink {
  public:
   explicit PlanSink(Plan &p) : plan_(p) {}
-  bool mfma_bconv() const override { return plan_.ctx->bconv_mfma != 0; }
+  bool mfma_bconv() const override { return plan_.bconv_mfma; }
   void begin_item() { cursor_ = 0; }
   bool mismatch() const { return mismatch_; }
   int pass(bool forward, int flags, const std::vector<PassJob> &jobs) override;
@@ -58,5 +62,8 @@ int plan_finish(Plan &p, bool mismatch);
 int plan_run(const Plan &p, hipStream_t st);
 void plan_destroy(Plan *p);
 int plan_launches(const Plan &p);
+// mean run time of a plan over `reps` runs after one warm-up, on a private
+// stream (host-synchronising; for plan creation only)
+double plan_time_us(const Plan &p, int reps);
 
 }  // namespace ck

@@ -288,7 +288,9 @@ int ck_plan_hrot_hoisted(ck_context *ctx, int n, const int *levels,
                          const uint32_t *const *key_a, uint32_t *const *out_b,
                          uint32_t *const *out_a, int moddown, ck_plan **out);
 int ck_plan_run(ck_plan *plan, void *stream);
-/* out[4]: items, steps, kernel launches per run, workspace bytes */
+/* out[8]: items, steps, kernel launches per run, workspace bytes, BaseConv
+ * form (1: matrix cores, 0: fused), and with autotuning the measured run
+ * time of each form in ns (fused, matrix cores; -1000 when not tuned), 0 */
 int ck_plan_info(const ck_plan *plan, long long *out);
 int ck_plan_destroy(ck_plan *plan);
 
@@ -300,6 +302,12 @@ int ck_plan_destroy(ck_plan *plan);
  * reference's kFuseModDownEpilogue-style compile-time switches,
  * include/core/Parameter.h:13-20, select kernel fusions the same way). */
 #define CK_OPT_BCONV_MFMA 1
+/* CK_OPT_PLAN_AUTOTUNE: 1 makes ck_plan_hmult / ck_plan_hrot_hoisted build
+ * the plan with both BaseConv forms, run each (one warm-up + 3 timed runs
+ * on a private stream, synchronising the host: plan creation only; the runs
+ * write the plan's outputs) and keep the faster; 0 (default) uses
+ * CK_OPT_BCONV_MFMA. */
+#define CK_OPT_PLAN_AUTOTUNE 2
 int ck_context_set_option(ck_context *ctx, int option, int value);
 
 /* ---- Per-kernel timing ---------------------------------------------------- */

@@ -21,10 +21,13 @@ def ck():
     return cheddar_amd
 
 
-@pytest.fixture(scope="module", params=["bconv-fused", "bconv-mfma"])
+@pytest.fixture(scope="module", params=["bconv-fused", "bconv-mfma", "autotune"])
 def ctx24(ck, rns24, request):
     ctx = ck.Context.from_json(rns24)
-    ctx.set_bconv_mfma(request.param == "bconv-mfma")
+    if request.param == "autotune":
+        ctx.set_plan_autotune(True)
+    else:
+        ctx.set_bconv_mfma(request.param == "bconv-mfma")
     return ctx
 
 
@@ -52,7 +55,12 @@ def test_plan_hmult_mixed_levels(ck, ctx24, oracle_rns24, key24):
     plan = ck.Plan.hmult(ctx24, levels, outs, d1, d2, dev_key(*key24))
     info = plan.info()
     # 9 launches (BaseConv fused) or 11 (two BaseConv steps on the matrix cores)
-    assert info["items"] == len(levels) and info["launches"] in (9, 11)
+    assert info["items"] == len(levels)
+    assert info["launches"] == (11 if info["bconv_mfma"] else 9)
+    if info["tuned_us"] is not None:  # autotuned: the kept form is the faster
+        fused, mfma = info["tuned_us"]
+        assert min(fused, mfma) > 0
+        assert info["bconv_mfma"] == (mfma < fused)
     for rep in range(2):
         for o in outs:
             o[0].fill_(-1)
