@@ -104,7 +104,9 @@ def _short(sym):
 
 
 _KERNELS = ("ntt_fwd_cols", "intt_cols", "ntt_fwd_rows", "intt_rows", "key_accum",
-            "key_accum_table", "bs_fused", "bs_fused_table", "tensor", "permute")
+            "key_accum_table", "bs_fused", "bs_fused_table", "tensor", "permute",
+            "binary", "unary", "const", "accum", "imag_unit", "crt_lift", "bconv_mfma",
+            "bconv_mfma_table")
 
 
 def _limbs(short, grid_x, grid_y, wg_x):
