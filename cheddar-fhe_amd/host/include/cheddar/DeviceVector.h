@@ -22,12 +22,9 @@
 #include <utility>
 #include <vector>
 
-namespace cheddar {
+#include "cheddar/Error.h"
 
-class Error : public std::runtime_error {
- public:
-  using std::runtime_error::runtime_error;
-};
+namespace cheddar {
 
 inline void HipCheck(hipError_t e, const char *what) {
   if (e != hipSuccess)

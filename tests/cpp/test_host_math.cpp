@@ -5,13 +5,19 @@
 //   returns slot j (the canonical embedding, brute force at N = 32);
 // - DoubleToResidue equals [trunc(x)]_q computed in 128-bit integers, for
 //   values up to 2^120;
-// - CRT reconstruction returns the centred integer (128-bit reference).
+// - CRT reconstruction returns the centred integer (128-bit reference);
+// - striped (diagonal-stored) matrix products equal dense products;
+// - the special-FFT stage matrices of bootstrapping compose to the encoder's
+//   special FFT and inverse (with the bit reversal and the 1/n they leave
+//   out).
 #include <cmath>
 #include <cstdio>
 #include <random>
 #include <vector>
 
 #include "cheddar/HostMath.h"
+#include "cheddar/extension/SpecialFFTMatrices.h"
+#include "cheddar/extension/StripedMatrix.h"
 
 using namespace cheddar;
 
@@ -96,6 +102,65 @@ int main() {
     if (std::fabs(got - want) > 1e-15 * std::fabs(want)) ok = false;
   }
   Check(ok, "CRT reconstruction returns the centred integer");
+
+  {
+    // striped product vs dense product
+    const int n = 16;
+    auto random_striped = [&](std::initializer_list<int> diags) {
+      StripedMatrix m(n, n);
+      for (int d : diags) {
+        std::vector<Complex> v(n);
+        for (auto &e : v) e = Complex(u(g), u(g));
+        m.emplace(d, v);
+      }
+      return m;
+    };
+    auto dense = [&](const StripedMatrix &m) {
+      std::vector<std::vector<Complex>> D(n, std::vector<Complex>(n));
+      for (const auto &[i, d] : m)
+        for (int k = 0; k < n; k++) D[k][(k + i) % n] += d[k];
+      return D;
+    };
+    const StripedMatrix a = random_striped({0, 3, 15}), b = random_striped({1, 2, 8});
+    const auto C = dense(StripedMatrix::Mult(a, b)), A = dense(a), B = dense(b);
+    double e = 0;
+    for (int r = 0; r < n; r++)
+      for (int c = 0; c < n; c++) {
+        Complex acc = 0;
+        for (int t = 0; t < n; t++) acc += A[r][t] * B[t][c];
+        e = std::fmax(e, std::abs(acc - C[r][c]));
+      }
+    Check(e < 1e-12, "striped matrix product equals the dense product");
+  }
+
+  {
+    // stage matrices compose to the special FFT pair
+    const int logn = 12, n = 1 << logn, M = 2 * n, slots = n / 2;
+    std::vector<Complex> tw(M);
+    for (int i = 0; i < M; i++) tw[i] = std::polar(1.0, 2.0 * M_PI * i / M);
+    std::vector<int> gal(slots + 1, 1);
+    for (int i = 1; i <= slots; i++) gal[i] = (int)((long long)gal[i - 1] * 5 % M);
+    std::vector<StripedMatrix> fft, ifft;
+    SpecialFFTStageMatrices(slots, M, gal, tw, fft, ifft);
+    std::vector<Complex> v(slots);
+    for (auto &x : v) x = Complex(u(g), u(g));
+    std::vector<Complex> y = v;
+    for (const auto &m : fft) y = StripedApply(m, y);
+    std::vector<Complex> want = v;
+    BitReverseVector(want);
+    SpecialFFT(want, tw, gal, M);
+    double e = 0;
+    for (int i = 0; i < slots; i++) e = std::fmax(e, std::abs(y[i] - want[i]));
+    Check(e < 1e-9, "forward stage matrices compose to SpecialFFT(bitrev(v))");
+    y = v;
+    for (const auto &m : ifft) y = StripedApply(m, y);
+    want = v;
+    SpecialIFFT(want, tw, gal, M);
+    BitReverseVector(want);
+    e = 0;
+    for (int i = 0; i < slots; i++) e = std::fmax(e, std::abs(y[i] - (double)slots * want[i]));
+    Check(e < 1e-9, "inverse stage matrices compose to n bitrev(SpecialIFFT(v))");
+  }
   std::printf("OK: %d failure(s)\n", fails);
   return fails ? 1 : 0;
 }
