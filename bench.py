@@ -485,7 +485,8 @@ def bench_boot(runs):
         f"bits (max error), {out['precision_bits_mean']:.2f} bits (mean); kernel sum "
         f"{total:.2f} ms; {out['graph_batch']} in one captured graph: "
         f"{out['graph_ms_per_bootstrap']:.2f} ms per bootstrap "
-        f"({out['graph_precision_bits_max']:.2f} bits)")
+        f"({out['graph_precision_bits_max']:.2f} bits); min-KS schedule: "
+        f"{out['minks_ms_per_bootstrap']:.2f} ms ({out['minks_precision_bits_max']:.2f} bits)")
     out["kernel_ms"] = total
     out["config"] = ("bootparam_40.json as shipped: N=2^16, 2^15 slots, dense secret "
                      "(Hamming weight 32768) with sparse-secret encapsulation (32)")
