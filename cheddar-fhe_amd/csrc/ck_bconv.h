@@ -28,9 +28,9 @@ struct BconvLaunch {
 };
 
 void launch_bconv_mfma(const Context &ctx, const BconvLaunch &L, hipStream_t st);
-// table form: n instances in device memory; dsts = total destination limbs;
-// small: every instance has D <= 32 and S <= 12
+// table form: n instances in device memory; dsts = total destination limbs,
+// max_s / max_d = the largest S and D among the instances
 void launch_bconv_mfma_table(const Context &ctx, const BconvInst *insts, int n,
-                             int dsts, bool small, hipStream_t st);
+                             int dsts, int max_s, int max_d, hipStream_t st);
 
 }  // namespace ck

@@ -21,6 +21,8 @@
 // the forward column pass reads.
 #include "ck_bconv.h"
 
+#include <algorithm>
+
 namespace ck {
 namespace {
 
@@ -33,27 +35,31 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int kTiles = kChunk / 4 / 16;  // 16-position tiles per wave
 
 // I: the instance, read in place (kernel argument or constant-space table);
-// RT / KS: static bounds on the row tiles (8 destinations each) and k-steps
-// (4 sources each) so that the operand arrays stay in registers
+// RT: static bound on the row tiles (8 destinations each), KS: the k-steps
+// (4 sources each) every instance of the launch is padded to, so that the
+// operand arrays stay in registers. Every load is unconditional (an index
+// past the instance's S or D is clamped to 0 and its value replaced by 0
+// afterwards): a load inside a branch is waited for before the branch ends,
+// which would serialise the source loads of a wave.
 template <int RT, int KS, class Inst>
 __device__ __forceinline__ void bconv_body(Inst &I, const PrimeTab &P, int chunk) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int g = lane >> 4, n16 = lane & 15;
   const int S = I.S, D = I.D;
-  const int nks = (S + 3) >> 2, nrt = (D + 7) >> 3;
+  const int nrt = (D + 7) >> 3;
   const int base = chunk * kChunk + wave * (kChunk / 4);
 
   // every source value this lane needs, loaded up front (one memory
   // latency per wave): source k = 4 ks + l / 16 at position base + 16 t + l % 16
   int32_t raw[kTiles][KS];
 #pragma unroll
-  for (int t = 0; t < kTiles; t++)
+  for (int ks = 0; ks < KS; ks++) {
+    const int k = ks * 4 + g;
+    const int32_t *sp = I.src + (size_t)(k < S ? k : 0) * kN + base + n16;
 #pragma unroll
-    for (int ks = 0; ks < KS; ks++) {
-      const int k = ks * 4 + g;
-      raw[t][ks] = (ks < nks && k < S) ? I.src[(size_t)k * kN + base + t * 16 + n16] : 0;
-    }
+    for (int t = 0; t < kTiles; t++) raw[t][ks] = sp[t * 16];
+  }
 
   // A operands of every (row tile, k-step): lane row m = l % 16 is
   // destination 2 (m % 4) + m / 8 of the tile, half (m / 4) % 2
@@ -65,9 +71,9 @@ __device__ __forceinline__ void bconv_body(Inst &I, const PrimeTab &P, int chunk
 #pragma unroll
     for (int ks = 0; ks < KS; ks++) {
       const int d = rt * 8 + dl, k = ks * 4 + g;
-      a[rt][ks] = (rt < nrt && ks < nks && d < D && k < S)
-                      ? I.tab[(d * S + k) * 2 + half]
-                      : 0.0;
+      const bool ok = d < D && k < S;
+      const double v = I.tab[(ok ? d * S + k : 0) * 2 + half];
+      a[rt][ks] = ok ? v : 0.0;
     }
 
   // per (row tile, e): this lane's destination 2g + e of the tile is the
@@ -79,13 +85,17 @@ __device__ __forceinline__ void bconv_body(Inst &I, const PrimeTab &P, int chunk
 #pragma unroll
     for (int e = 0; e < 2; e++) {
       const int d = rt * 8 + 2 * g + e;
-      const bool ok = rt < nrt && d < D;
-      const int p = ok ? I.dst_prime[d] : 0;
+      const bool ok = d < D;
+      const int dd = ok ? d : 0;
+      const int p = I.dst_prime[dd];
+      const int limb = I.dst_limb[dd];
       eq[rt][e] = P.qd[p];
       eqr[rt][e] = P.qr[p];
-      edst[rt][e] = ok ? I.dst + (size_t)I.dst_limb[d] * kN : nullptr;
+      edst[rt][e] = ok ? I.dst + (size_t)limb * kN : nullptr;
     }
 
+  // sources past S have zero table entries (a = 0), so their k-step
+  // products vanish and every instance runs the same KS steps
 #pragma unroll
   for (int t = 0; t < kTiles; t++) {
     const int pos = base + t * 16 + n16;
@@ -97,10 +107,8 @@ __device__ __forceinline__ void bconv_body(Inst &I, const PrimeTab &P, int chunk
       if (rt >= nrt) break;
       v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int ks = 0; ks < KS; ks++) {
-        if (ks >= nks) break;
+      for (int ks = 0; ks < KS; ks++)
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt][ks], b[ks], acc, 0, 0, 0);
-      }
 #pragma unroll
       for (int e = 0; e < 2; e++) {
         if (edst[rt][e]) {
@@ -115,8 +123,6 @@ __device__ __forceinline__ void bconv_body(Inst &I, const PrimeTab &P, int chunk
   }
 }
 
-// small instances (D <= 32, S <= 12: every ModUp and ModDown of the 24+8
-// configuration) and the general bound
 template <int RT, int KS>
 __global__ __launch_bounds__(kThreads) void bconv_mfma_kernel(const BconvLaunch L,
                                                               const PrimeTab P) {
@@ -131,34 +137,50 @@ __global__ __launch_bounds__(kThreads) void bconv_mfma_table_kernel(
   bconv_body<RT, KS>(I, P, blockIdx.x);
 }
 
+// the instantiation for a launch whose instances have at most max_s sources
+// and max_d destinations: row tiles 4 (D <= 32) or 8, k-steps 2, 3 or 4
+template <int RT, class F>
+void pick_ks(int max_s, F &&f) {
+  const int ks = (max_s + 3) >> 2;
+  if (ks <= 2)
+    f(bconv_mfma_kernel<RT, 2>, bconv_mfma_table_kernel<RT, 2>);
+  else if (ks == 3)
+    f(bconv_mfma_kernel<RT, 3>, bconv_mfma_table_kernel<RT, 3>);
+  else
+    f(bconv_mfma_kernel<RT, 4>, bconv_mfma_table_kernel<RT, 4>);
+}
+
+template <class F>
+void pick(int max_s, int max_d, F &&f) {
+  if (max_d <= 32)
+    pick_ks<4>(max_s, f);
+  else
+    pick_ks<kMaxMfmaRowTiles>(max_s, f);
+}
+
 }  // namespace
 
 void launch_bconv_mfma(const Context &ctx, const BconvLaunch &L, hipStream_t st) {
   if (L.n <= 0) return;
-  int dsts = 0;
-  bool small = true;
+  int dsts = 0, max_s = 1, max_d = 1;
   for (int i = 0; i < L.n; i++) {
     dsts += L.inst[i].D;
-    small = small && L.inst[i].D <= 32 && L.inst[i].S <= 12;
+    max_s = std::max(max_s, L.inst[i].S);
+    max_d = std::max(max_d, L.inst[i].D);
   }
-  if (small)
-    launch(ctx, "bconv_mfma", dsts, bconv_mfma_kernel<4, 3>, dim3(kN / kChunk, L.n),
-           dim3(kThreads), st, L, ctx.ptab);
-  else
-    launch(ctx, "bconv_mfma", dsts, bconv_mfma_kernel<kMaxMfmaRowTiles, kMaxMfmaKSteps>,
-           dim3(kN / kChunk, L.n), dim3(kThreads), st, L, ctx.ptab);
+  pick(max_s, max_d, [&](auto kern, auto) {
+    launch(ctx, "bconv_mfma", dsts, kern, dim3(kN / kChunk, L.n), dim3(kThreads), st, L,
+           ctx.ptab);
+  });
 }
 
 void launch_bconv_mfma_table(const Context &ctx, const BconvInst *insts, int n,
-                             int dsts, bool small, hipStream_t st) {
+                             int dsts, int max_s, int max_d, hipStream_t st) {
   if (n <= 0) return;
-  if (small)
-    launch(ctx, "bconv_mfma[batch]", dsts, bconv_mfma_table_kernel<4, 3>,
-           dim3(kN / kChunk, n), dim3(kThreads), st, insts, ctx.ptab);
-  else
-    launch(ctx, "bconv_mfma[batch]", dsts,
-           bconv_mfma_table_kernel<kMaxMfmaRowTiles, kMaxMfmaKSteps>,
-           dim3(kN / kChunk, n), dim3(kThreads), st, insts, ctx.ptab);
+  pick(max_s, max_d, [&](auto, auto kern) {
+    launch(ctx, "bconv_mfma[batch]", dsts, kern, dim3(kN / kChunk, n), dim3(kThreads), st,
+           insts, ctx.ptab);
+  });
 }
 
 }  // namespace ck

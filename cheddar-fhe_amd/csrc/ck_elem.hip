@@ -29,12 +29,31 @@ __device__ __forceinline__ size_t ew_off(const EwOperand &o, const EwNp &np,
   const int32_t qinv = P.qinv[p];                      \
   (void)qinv;
 
+// every operand is device memory: global (not flat) accesses, also through
+// pointers that a batched plan's table holds
+// (a native vector type: HIP's uint4 copy goes through a generic reference)
+typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) v4u32 gv4u32;
 __device__ __forceinline__ uint4 ld4(const uint32_t *p) {
-  return *reinterpret_cast<const uint4 *>(p);
+  const v4u32 v = *(const gv4u32 *)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
 }
 __device__ __forceinline__ void st4(uint32_t *p, uint4 v) {
-  *reinterpret_cast<uint4 *>(p) = v;
+  const v4u32 w = {v.x, v.y, v.z, v.w};
+  *(gv4u32 *)p = w;
 }
+__device__ __forceinline__ uint32_t ld1(const uint32_t *p) {
+  return *(const __attribute__((address_space(1))) uint32_t *)p;
+}
+__device__ __forceinline__ void st1(uint32_t *p, uint32_t v) {
+  *(__attribute__((address_space(1))) uint32_t *)p = v;
+}
+
+// compile-time count tag for the per-count bodies below
+template <int N>
+struct IntC {
+  static constexpr int value = N;
+};
 
 template <typename F>
 __device__ __forceinline__ uint4 map2(uint4 a, uint4 b, F f) {
@@ -245,16 +264,18 @@ __device__ __forceinline__ void key_accum_body(const EwNp &np,
       d2[i] = mont_mul(A1[i], A2[i], q, qinv);
     }
   }
+  // one group at a time: the kernel is HBM-bound with many waves per SIMD,
+  // which hide each group's load latency; batching the groups' loads (more
+  // registers, fewer waves) measured slower
   for (int gi = 0; gi < A.ngroups; gi++) {
     const int g = A.group[gi];
     const bool own = (y >= A.own0[g] && y < A.own1[g]);
-    uint4 mu;
-    if (own)
-      mu = tens ? make_uint4(d2[0], d2[1], d2[2], d2[3]) : ld4(A.x + my);
-    else
-      mu = ld4(A.modup + (size_t)g * tot_stride + my);
-    uint4 kb = ld4(A.key_b[g] + kp);
-    uint4 ka = ld4(A.key_a[g] + kp);
+    const uint32_t *src = own ? (tens ? A.tensor[1] : A.x)
+                              : A.modup + (size_t)g * tot_stride;
+    uint4 mu = ld4(src + my);
+    const uint4 kb = ld4(A.key_b[g] + kp);
+    const uint4 ka = ld4(A.key_a[g] + kp);
+    if (own && tens) mu = make_uint4(d2[0], d2[1], d2[2], d2[3]);
     const uint32_t M[4] = {mu.x, mu.y, mu.z, mu.w};
     const uint32_t KB[4] = {kb.x, kb.y, kb.z, kb.w};
     const uint32_t KA[4] = {ka.x, ka.y, ka.z, ka.w};
@@ -265,14 +286,14 @@ __device__ __forceinline__ void key_accum_body(const EwNp &np,
     }
   }
   if (tens) {
-    const uint32_t pp = A.p_prod[p];
+    const uint32_t pp = ld1(A.p_prod + p);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       rb[i] = add_mod(rb[i], mont_mul(pp, d0[i], q, qinv), q);
       ra[i] = add_mod(ra[i], mont_mul(pp, d1[i], q, qinv), q);
     }
   } else if (y < np.num_q) {
-    const uint32_t pp = A.p_prod[p];
+    const uint32_t pp = ld1(A.p_prod + p);
     if (A.add_b) {
       uint4 v = ld4(A.add_b + my);
       const uint32_t V[4] = {v.x, v.y, v.z, v.w};
@@ -325,21 +346,40 @@ __device__ __forceinline__ void bs_fused_body(const EwNp &np, const PrimeTab &P,
   const size_t kp = (size_t)p * 65536 + k;
   const size_t tot_stride = (size_t)np.num_total * 65536;
   // register arrays: every loop over groups is unrolled to the static bound
+  // loads of all groups issued before any use (a load in a branch that
+  // also uses it is waited for at once): exact bodies per count up to 4
   uint32_t mu[kMaxBsGroups][4];
+  auto load_mu = [&](auto ng_tag, int g0) {
+    constexpr int NG = decltype(ng_tag)::value;
+    uint4 v[NG];
 #pragma unroll
-  for (int gi = 0; gi < kMaxBsGroups; gi++) {
-    if (gi >= A.ngroups) break;
-    const bool own = (y >= A.own0[gi] && y < A.own1[gi]);
-    const uint4 v = own ? ld4(A.x + my)
-                        : ld4(A.modup + (size_t)A.group[gi] * tot_stride + my);
-    mu[gi][0] = v.x;
-    mu[gi][1] = v.y;
-    mu[gi][2] = v.z;
-    mu[gi][3] = v.w;
+    for (int b = 0; b < NG; b++) {
+      const int gi = g0 + b;
+      const bool own = (y >= A.own0[gi] && y < A.own1[gi]);
+      const uint32_t *src = own ? A.x : A.modup + (size_t)A.group[gi] * tot_stride;
+      v[b] = ld4(src + my);
+    }
+#pragma unroll
+    for (int b = 0; b < NG; b++) {
+      mu[g0 + b][0] = v[b].x;
+      mu[g0 + b][1] = v[b].y;
+      mu[g0 + b][2] = v[b].z;
+      mu[g0 + b][3] = v[b].w;
+    }
+  };
+  const int ng = A.ngroups;
+  const int nfull = ng >= 4 ? 4 : 0;  // kMaxBsGroups = 8: at most two batches
+  if (nfull) load_mu(IntC<4>{}, 0);
+  switch (ng - nfull) {
+    case 1: load_mu(IntC<1>{}, nfull); break;
+    case 2: load_mu(IntC<2>{}, nfull); break;
+    case 3: load_mu(IntC<3>{}, nfull); break;
+    case 4: load_mu(IntC<4>{}, nfull); break;
+    default: break;
   }
   uint32_t pb[4] = {0, 0, 0, 0};
   if (y < np.num_q) {
-    const uint32_t pp = A.p_prod[p];
+    const uint32_t pp = ld1(A.p_prod + p);
     const uint4 v = ld4(A.b + my);
     pb[0] = mont_mul(pp, v.x, q, qinv);
     pb[1] = mont_mul(pp, v.y, q, qinv);
@@ -349,18 +389,32 @@ __device__ __forceinline__ void bs_fused_body(const EwNp &np, const PrimeTab &P,
   uint32_t d[4];
   for (int r = 0; r < A.nrot; r++) {
     uint32_t rb[4] = {pb[0], pb[1], pb[2], pb[3]}, ra[4] = {0, 0, 0, 0};
+    auto keys = [&](auto ng_tag, int g0) {
+      constexpr int NG = decltype(ng_tag)::value;
+      uint4 kb[NG], ka[NG];
 #pragma unroll
-    for (int gi = 0; gi < kMaxBsGroups; gi++) {
-      if (gi >= A.ngroups) break;
-      const uint4 kb = ld4(A.key_b[r][gi] + kp);
-      const uint4 ka = ld4(A.key_a[r][gi] + kp);
-      const uint32_t KB[4] = {kb.x, kb.y, kb.z, kb.w};
-      const uint32_t KA[4] = {ka.x, ka.y, ka.z, ka.w};
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        rb[i] = add_mod(rb[i], mont_mul(mu[gi][i], KB[i], q, qinv), q);
-        ra[i] = add_mod(ra[i], mont_mul(mu[gi][i], KA[i], q, qinv), q);
+      for (int b = 0; b < NG; b++) {
+        kb[b] = ld4(A.key_b[r][g0 + b] + kp);
+        ka[b] = ld4(A.key_a[r][g0 + b] + kp);
       }
+#pragma unroll
+      for (int b = 0; b < NG; b++) {
+        const uint32_t KB[4] = {kb[b].x, kb[b].y, kb[b].z, kb[b].w};
+        const uint32_t KA[4] = {ka[b].x, ka[b].y, ka[b].z, ka[b].w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          rb[i] = add_mod(rb[i], mont_mul(mu[g0 + b][i], KB[i], q, qinv), q);
+          ra[i] = add_mod(ra[i], mont_mul(mu[g0 + b][i], KA[i], q, qinv), q);
+        }
+      }
+    };
+    if (nfull) keys(IntC<4>{}, 0);
+    switch (ng - nfull) {
+      case 1: keys(IntC<1>{}, nfull); break;
+      case 2: keys(IntC<2>{}, nfull); break;
+      case 3: keys(IntC<3>{}, nfull); break;
+      case 4: keys(IntC<4>{}, nfull); break;
+      default: break;
     }
     const uint32_t g = A.ginv[r];
 #pragma unroll
@@ -372,8 +426,8 @@ __device__ __forceinline__ void bs_fused_body(const EwNp &np, const PrimeTab &P,
     uint32_t *oa = A.out_a[r] + (size_t)y * 65536;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      ob[d[i]] = rb[i];
-      oa[d[i]] = ra[i];
+      st1(ob + d[i], rb[i]);
+      st1(oa + d[i], ra[i]);
     }
   }
 }

@@ -188,49 +188,93 @@ __device__ __forceinline__ ColTile col_tile() {
 // T_j = hi_j * 2^16 + lo_j so every partial sum stays an exact double
 // (|v| < 2^30: |sum v hi| < 2^49, |sum v lo| < 2^50 for bs <= 16).
 // Output: int32 tile [256][16] in LDS, 16-byte source loads.
+//
+// MS (>= bs) bounds the unrolled source loop. All loads
+// of a batch of up to KB sources are issued unconditionally before the first
+// product (sources past bs re-read source 0 and get zero table entries): a
+// load inside a branch that also consumes it would be waited for at once,
+// one memory latency per source. VEC coefficients per load: 4 (16 bytes),
+// or 2 for MS >= 12, whose 24 or 32 table doubles leave fewer registers.
+template <int MS, int VEC, int KB = 8>
+__device__ __forceinline__ void bconv_cols_ms(const int32_t *bsrc, int bs,
+                                              const double *tab, int c0,
+                                              const ModF &m, int32_t *lds) {
+  typedef int32_t ivec __attribute__((ext_vector_type(VEC)));
+  constexpr int kRowLanes = kColTile / VEC;                 // lanes per row
+  constexpr int kIters = 256 * kColTile / (kColThreads * VEC);
+  // the table row is read-only and uniform: scalar loads (constant space)
+  typedef const __attribute__((address_space(4))) double cdouble;
+  cdouble *ctab = (cdouble *)tab;
+  double th[MS], tl[MS];
+#pragma unroll
+  for (int j = 0; j < MS; j++) {
+    const int jj = j < bs ? j : 0;
+    const double h = ctab[2 * jj], l = ctab[2 * jj + 1];
+    th[j] = j < bs ? h : 0.0;
+    tl[j] = j < bs ? l : 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < kIters; i++) {
+    const int idx = kColThreads * i + threadIdx.x;  // VEC-word chunk index
+    const int r = idx / kRowLanes, cv = (idx % kRowLanes) * VEC;
+    const int32_t *p = bsrc + r * 256 + c0 + cv;
+    double sh[VEC], sl[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; e++) sh[e] = sl[e] = 0.0;
+#pragma unroll
+    for (int j0 = 0; j0 < MS; j0 += KB) {
+      constexpr int kB = MS < KB ? MS : KB;
+      ivec v[kB];
+#pragma unroll
+      for (int j = 0; j < kB; j++) {
+        if (j0 + j >= MS) break;  // static after unrolling
+        const int jj = j0 + j < bs ? j0 + j : 0;
+        v[j] = *reinterpret_cast<const ivec *>(p + (size_t)jj * kN);
+      }
+#pragma unroll
+      for (int j = 0; j < kB; j++) {
+        if (j0 + j >= MS) break;
+#pragma unroll
+        for (int e = 0; e < VEC; e++) {
+          const double vv = (double)v[j][e];
+          sh[e] = __builtin_fma(vv, th[j0 + j], sh[e]);
+          sl[e] = __builtin_fma(vv, tl[j0 + j], sl[e]);
+        }
+      }
+    }
+    ivec o;
+#pragma unroll
+    for (int e = 0; e < VEC; e++) {
+      const double big = sh[e] * 65536.0;  // exact (power-of-two scale)
+      const double k = __builtin_rint(big * m.qinv);
+      o[e] = (int32_t)reduce_f(__builtin_fma(-k, m.q, big) + sl[e], m);
+    }
+    *reinterpret_cast<ivec *>(lds + r * kColTile + cv) = o;
+  }
+}
+
+// bs is uniform per workgroup: one branch-free body per source bound
 __device__ __forceinline__ void bconv_cols_to_lds(const int32_t *bsrc, int bs,
                                                   const double *tab, int c0,
                                                   const ModF &m,
                                                   int32_t *lds) {
-  // the table row is read-only and uniform: scalar loads (constant space)
-  typedef const __attribute__((address_space(4))) double cdouble;
-  cdouble *ctab = (cdouble *)tab;
-  double th[kMaxBconvSrc], tl[kMaxBconvSrc];
-#pragma unroll
-  for (int j = 0; j < kMaxBconvSrc; j++) {
-    if (j < bs) {
-      th[j] = ctab[2 * j];
-      tl[j] = ctab[2 * j + 1];
-    }
+  // exact bodies for the common counts (alpha and 1 + alpha of the shipped
+  // parameter sets: 8, 9, 12, 13), bounded bodies otherwise
+  switch (bs) {
+    case 8: bconv_cols_ms<8, 4>(bsrc, bs, tab, c0, m, lds); return;
+    case 9: bconv_cols_ms<9, 4, 5>(bsrc, bs, tab, c0, m, lds); return;
+    case 12: bconv_cols_ms<12, 2>(bsrc, bs, tab, c0, m, lds); return;
+    case 13: bconv_cols_ms<13, 2>(bsrc, bs, tab, c0, m, lds); return;
+    default: break;
   }
-  for (int i = 0; i < 4; i++) {
-    const int idx = kColThreads * i + threadIdx.x;  // 16-byte chunk index
-    const int r = idx >> 2, c4 = (idx & 3) * 4;
-    const int32_t *p = bsrc + r * 256 + c0 + c4;
-    double sh[4] = {0.0, 0.0, 0.0, 0.0}, sl[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int j = 0; j < kMaxBconvSrc; j++) {
-      if (j < bs) {
-        const int4 v = *reinterpret_cast<const int4 *>(p + (size_t)j * kN);
-        const double vv[4] = {(double)v.x, (double)v.y, (double)v.z,
-                              (double)v.w};
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-          sh[e] = __builtin_fma(vv[e], th[j], sh[e]);
-          sl[e] = __builtin_fma(vv[e], tl[j], sl[e]);
-        }
-      }
-    }
-    int4 o;
-    int32_t *op = reinterpret_cast<int32_t *>(&o);
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const double big = sh[e] * 65536.0;  // exact (power-of-two scale)
-      const double k = __builtin_rint(big * m.qinv);
-      op[e] = (int32_t)reduce_f(__builtin_fma(-k, m.q, big) + sl[e], m);
-    }
-    *reinterpret_cast<int4 *>(lds + r * kColTile + c4) = o;
-  }
+  if (bs <= 4)
+    bconv_cols_ms<4, 4>(bsrc, bs, tab, c0, m, lds);
+  else if (bs <= 8)
+    bconv_cols_ms<8, 4>(bsrc, bs, tab, c0, m, lds);
+  else if (bs <= 12)
+    bconv_cols_ms<12, 2>(bsrc, bs, tab, c0, m, lds);
+  else
+    bconv_cols_ms<kMaxBconvSrc, 2>(bsrc, bs, tab, c0, m, lds);
 }
 
 // Final value of an element: canonical u32, or lazy int32 in
@@ -260,22 +304,27 @@ __global__ __launch_bounds__(kColThreads, 5) void ntt_fwd_cols(
   // phase 1 twiddles psi_rev[2^s + blk] (uniform), phase 2 psi_rev[2^(4+s)
   // + rb 2^s + blk]
   uint32_t t1[15], t2[15];
+  auto load_twiddles = [&]() {
 #pragma unroll
-  for (int s = 0; s < 4; s++)
+    for (int s = 0; s < 4; s++)
 #pragma unroll
-    for (int blk = 0; blk < (1 << s); blk++) {
-      t1[(1 << s) - 1 + blk] = w[(1 << s) + blk];
-      t2[(1 << s) - 1 + blk] = w[(16 << s) + (rb << s) + blk];
-    }
+      for (int blk = 0; blk < (1 << s); blk++) {
+        t1[(1 << s) - 1 + blk] = w[(1 << s) + blk];
+        t2[(1 << s) - 1 + blk] = w[(16 << s) + (rb << s) + blk];
+      }
+  };
   // phase 1 holds rows rb + 16 k of column c0 + cc
   double x[16];
   if (J.flags & kNttBconvInput) {
+    // twiddles after the BaseConv, whose loads need the registers
     bconv_cols_to_lds(J.bsrc, J.bs, J.btab, c0, m, lds);
+    load_twiddles();
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 16; k++) x[k] = (double)lds[(rb + 16 * k) * kColTile + cc];
     __syncthreads();  // the staging words are reused by the double tile
   } else {
+    load_twiddles();
     // canonical residues (< 2^31) or a lazy signed BaseConv output: int32
     const int32_t *__restrict__ g =
         reinterpret_cast<const int32_t *>(J.src) + c0 + cc;

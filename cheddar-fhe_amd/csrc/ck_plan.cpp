@@ -178,13 +178,13 @@ int plan_run(const Plan &p, hipStream_t st) {
         launch_bs_fused_table(c, s.d_bs, s.d_ew, (int)s.ew.size(), st);
         break;
       case PlanStage::kBconv: {
-        int dsts = 0;
-        bool small = true;
+        int dsts = 0, max_s = 1, max_d = 1;
         for (const BconvInst &I : s.bc) {
           dsts += I.D;
-          small = small && I.D <= 32 && I.S <= 12;
+          max_s = std::max(max_s, I.S);
+          max_d = std::max(max_d, I.D);
         }
-        launch_bconv_mfma_table(c, s.d_bc, (int)s.bc.size(), dsts, small, st);
+        launch_bconv_mfma_table(c, s.d_bc, (int)s.bc.size(), dsts, max_s, max_d, st);
         break;
       }
     }
