@@ -67,7 +67,7 @@ def alg_limbs(ctx, level, name):
         return 2 * n, n
     if kern == "tensor":
         return 4 * num_q, 3 * num_q
-    if kern == "key_accum":                        # ModUp out + ct1, ct2 + key
+    if kern in ("key_accum", "ntt_fwd_rows+key"):  # ModUp out + ct1, ct2 + key
         ext = num_q + alpha
         return (beta * ext - num_q) + 4 * num_q + 2 * beta * ext, 2 * ext
     raise ValueError(f"no byte model for kernel {name}")
@@ -77,6 +77,8 @@ def kernel_key(name):
     """'ntt_fwd_cols+bconv/72' -> 'ntt_fwd_cols/72': the kernel symbol and
     its limb count, the key rocprof summaries use."""
     kern, n = name.rsplit("/", 1)
+    if kern == "ntt_fwd_rows+key":
+        return f"key_rows/{n}"
     return f"{kern.split('+')[0]}/{n}"
 
 
@@ -106,7 +108,7 @@ def _short(sym):
 _KERNELS = ("ntt_fwd_cols", "intt_cols", "ntt_fwd_rows", "intt_rows", "key_accum",
             "key_accum_table", "bs_fused", "bs_fused_table", "tensor", "permute",
             "binary", "unary", "const", "accum", "imag_unit", "crt_lift", "bconv_mfma",
-            "bconv_mfma_table")
+            "bconv_mfma_table", "key_rows", "key_rows_table")
 
 
 def _limbs(short, grid_x, grid_y, wg_x):

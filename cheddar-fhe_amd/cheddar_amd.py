@@ -18,6 +18,7 @@ PARAM_DIR = os.path.join(HERE, "params")
 CK_MODDOWN, CK_RESCALE, CK_MODDOWN_RESCALE = 0, 1, 2
 CK_OPT_BCONV_MFMA = 1
 CK_OPT_PLAN_AUTOTUNE = 2
+CK_OPT_FUSED_KEY_ROWS = 3
 
 _vp = ctypes.c_void_p
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -376,6 +377,12 @@ class Context:
         batched plans only); outputs are identical."""
         v = -1 if on is None else int(bool(on))
         _check(lib.ck_context_set_option(self._h, CK_OPT_BCONV_MFMA, v))
+
+    def set_fused_key_rows(self, on=True):
+        """Key switches run the ModUp's forward row stages and the key inner
+        product as one pass (True, default) or as a row pass plus a separate
+        accumulation (False); outputs are identical."""
+        _check(lib.ck_context_set_option(self._h, CK_OPT_FUSED_KEY_ROWS, int(bool(on))))
 
     # ---- per-kernel timing --------------------------------------------------
     def profile(self, on=True):

@@ -755,6 +755,10 @@ int ck_context_set_option(ck_context *ctx, int option, int value) {
       CK_REQUIRE(value == 0 || value == 1, "option value must be 0 or 1");
       ctx->c->plan_autotune = value != 0;
       return CK_OK;
+    case CK_OPT_FUSED_KEY_ROWS:
+      CK_REQUIRE(value == 0 || value == 1, "option value must be 0 or 1");
+      ctx->c->fuse_key_rows = value != 0;
+      return CK_OK;
     default:
       return fail(CK_ERR_INVALID_ARG, "unknown context option");
   }

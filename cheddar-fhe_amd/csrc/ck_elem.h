@@ -144,6 +144,13 @@ void launch_key_accum_table(const Context &ctx, const KeyAccumItem *items,
                             const EwJob *jobs, int njobs, hipStream_t st);
 void launch_bs_fused_table(const Context &ctx, const BsFusedItem *items,
                            const EwJob *jobs, int njobs, hipStream_t st);
+// forward row stages of a ModUp fused with the key inner product (the
+// column pass ran with kNttColsOnly; A.modup holds its lazy int32 output);
+// writes A.out_b / A.out_a exactly as launch_key_accum would (ck_ntt.hip)
+void launch_key_rows(const Context &ctx, const EwNp &np, const KeyAccumArgs &A,
+                     hipStream_t st);
+void launch_key_rows_table(const Context &ctx, const KeyAccumItem *items,
+                           const EwJob *jobs, int njobs, hipStream_t st);
 
 inline EwNp make_np(const Context &c, int num_main, int num_ter, int num_aux) {
   EwNp np;

@@ -44,6 +44,7 @@ enum NttFlags : int {
   kInttCentred = 8,         // inverse: centred int32 output
   kInttLoadProduct = 16,    // inverse: input = src (*) src2 (Montgomery)
   kNttBconvInput = 32,      // forward: input = BaseConv of bsrc (see below)
+  kNttColsOnly = 64,        // forward: column stages only (rows fused later)
 };
 
 // Per-limb fields of a kernel-argument plan, kept together in one record
@@ -159,6 +160,10 @@ struct Context {
   int bconv_mfma = -1;
   // batched plans: build both BaseConv forms, time them, keep the faster
   bool plan_autotune = false;
+  // key switching: the ModUp's forward row stages fused with the key inner
+  // product (one pass, no NTT-form ModUp output) instead of a row pass and
+  // a separate key_accum
+  bool fuse_key_rows = true;
 
   // workspace for multi-kernel operations (one operation at a time per
   // context; concurrent calls on different streams race)

@@ -28,6 +28,7 @@
 // int32 in (-0.75 q, 0.75 q) and only final outputs are made canonical.
 // Each thread writes a phase's results back to exactly the LDS words it read
 // for that phase, so only the hand-offs between threads need a barrier.
+#include "ck_elem.h"
 #include "ck_internal.h"
 
 namespace ck {
@@ -51,6 +52,31 @@ __device__ __forceinline__ void ct16(double x[16], const uint32_t tw[15],
 #pragma unroll
     for (int blk = 0; blk < (1 << s); blk++) {
       const double w = (double)tw[(1 << s) - 1 + blk];
+      const double wq = w * qr;
+#pragma unroll
+      for (int j = 0; j < half; j++) {
+        double &a = x[blk * 2 * half + j];
+        double &b = x[blk * 2 * half + j + half];
+        const double r = mulmod_f(b, w, wq, q);
+        b = a - r;
+        a = a + r;
+      }
+    }
+  }
+}
+
+// ct16 with the twiddles kept in LDS, slot i of this thread at
+// tws[i * kRowThreads + threadIdx.x] (conflict-free): for a loop that runs
+// several transforms with the same twiddles and cannot keep them, and their
+// FP64 forms, in registers.
+__device__ __forceinline__ void ct16_lds(double x[16], const uint32_t *tws,
+                                         double q, double qr) {
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const int half = 8 >> s;
+#pragma unroll
+    for (int blk = 0; blk < (1 << s); blk++) {
+      const double w = (double)tws[((1 << s) - 1 + blk) * 256 + threadIdx.x];
       const double wq = w * qr;
 #pragma unroll
       for (int j = 0; j < half; j++) {
@@ -569,6 +595,205 @@ __global__ __launch_bounds__(kColThreads, 5) void intt_cols(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Forward row stages fused with the key inner product (MultKeyNoModDown,
+// src/core/Context.cpp:609-712: the ModUp's last NTT stages, then
+// ElementWise::PAccum / CAccum, src/core/ElementWise.cu:225-366)
+// ---------------------------------------------------------------------------
+//
+// One workgroup owns a 16-row tile of output limb y (prime p). For every
+// ModUp group g it takes limb y of that group's ModUp output straight from
+// the column pass (lazy int32 in `modup`), runs the row stages in registers
+// and multiplies the 16 values each thread holds by the key limbs, so the
+// ModUp output never goes back to memory in NTT form. A group's own limbs
+// are the ModUp source itself (x, or the Tensor product a1 (*) a2). In the
+// Montgomery convention of the stored data the result is
+//   acc_x[y] = R^-1 (sum_g m_g * key_x,g + fold_x)  mod q,
+// fold_b = P * b1 b2 (HMult) or P R * add_b, fold_a likewise, which is
+// exactly the key_accum kernel's sum of Montgomery products; here every
+// product is an exact FP64 modular product of lazy values and one product by
+// R^-1 ends the sum.
+template <class Args>
+__device__ __forceinline__ void key_rows_body(const EwNp &np, const PrimeTab &P,
+                                              Args &A, const uint32_t *tw, int y,
+                                              double *ldsd, uint32_t *tws) {
+  int32_t *lds = reinterpret_cast<int32_t *>(ldsd);
+  const int p = y < np.num_q ? np.q_prime0 + y : np.aux_prime0 + (y - np.num_q);
+  const double q = P.qd[p], qr = P.qr[p];
+  const uint32_t qu = P.q[p];
+  const int32_t qinv = P.qinv[p];
+  const uint32_t *__restrict__ w = tw + (size_t)p * kN;
+  const int rl = threadIdx.x >> 4;
+  const int t16 = threadIdx.x & 15;
+  const int r = blockIdx.x * kRowTile + rl;
+  const size_t tile = (size_t)blockIdx.x * kRowTile * 256;
+  const size_t tot_stride = (size_t)np.num_total * kN;
+  const size_t off = (size_t)r * 256 + t16 * 16;  // this thread's 16 outputs
+  const size_t my = (size_t)y * kN + off;
+  const size_t kp = (size_t)p * kN + off;
+  const bool tens = A.use_tensor && y < np.num_q;
+  // R^-1 and P = (P R) R^-1 as plain residues (Montgomery reductions)
+  const double rinv = (double)mont_reduce(1, qu, qinv), rinvq = rinv * qr;
+
+  // row-stage twiddles (as ntt_fwd_rows), parked in LDS for the group loop:
+  // slot i of this thread at tws[i * 256 + tid] (phase 1: 0..14, 2: 15..29);
+  // only this thread reads its slots back
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int blk = 0; blk < (1 << s); blk++) {
+      tws[((1 << s) - 1 + blk) * kRowThreads + threadIdx.x] = w[(256 << s) + (r << s) + blk];
+      tws[(15 + (1 << s) - 1 + blk) * kRowThreads + threadIdx.x] =
+          w[(4096 << s) + (r << (4 + s)) + (t16 << s) + blk];
+    }
+
+  double sb[16], sa[16];
+#pragma unroll
+  for (int j = 0; j < 16; j++) sb[j] = sa[j] = 0.0;
+  for (int gi = 0; gi < A.ngroups; gi++) {
+    const int g = A.group[gi];
+    const bool own = (y >= A.own0[g] && y < A.own1[g]);
+    // key limbs first: their loads overlap the row stages
+    uint4 kb[4], ka[4];
+    const uint32_t *kbp = A.key_b[g] + kp, *kap = A.key_a[g] + kp;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      kb[i] = reinterpret_cast<const uint4 *>(kbp)[i];
+      ka[i] = reinterpret_cast<const uint4 *>(kap)[i];
+    }
+    double x[16];
+    if (own) {
+      if (tens) {
+        // d2 = a1 (*) a2 = a1 a2 R^-1
+        const uint4 *a1 = reinterpret_cast<const uint4 *>(A.tensor[1] + my);
+        const uint4 *a2 = reinterpret_cast<const uint4 *>(A.tensor[3] + my);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const uint4 u = a1[i], v = a2[i];
+          const uint32_t U[4] = {u.x, u.y, u.z, u.w}, V[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+          for (int e = 0; e < 4; e++) {
+            const double vd = (double)V[e];
+            const double t = mulmod_f((double)U[e], vd, vd * qr, q);
+            x[4 * i + e] = mulmod_f(t, rinv, rinvq, q);
+          }
+        }
+      } else {
+        const uint4 *xs = reinterpret_cast<const uint4 *>(A.x + my);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const uint4 u = xs[i];
+          x[4 * i + 0] = (double)u.x;
+          x[4 * i + 1] = (double)u.y;
+          x[4 * i + 2] = (double)u.z;
+          x[4 * i + 3] = (double)u.w;
+        }
+      }
+    } else {
+      __syncthreads();  // the previous group's tile may still be read
+      tile_rows_to_lds(A.modup + (size_t)g * tot_stride + (size_t)y * kN + tile, lds);
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 16; k++) x[k] = (double)lds[rlds(rl, t16 + 16 * k)];
+      ct16_lds(x, tws, q, qr);
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 16; k++) ldsd[rlds(rl, t16 + 16 * k)] = x[k];
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < 16; j++) x[j] = ldsd[rlds(rl, 16 * t16 + j)];
+      ct16_lds(x, tws + 15 * kRowThreads, q, qr);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const uint32_t KB[4] = {kb[i].x, kb[i].y, kb[i].z, kb[i].w};
+      const uint32_t KA[4] = {ka[i].x, ka[i].y, ka[i].z, ka[i].w};
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const double b = (double)KB[e], a = (double)KA[e];
+        sb[4 * i + e] += mulmod_f(x[4 * i + e], b, b * qr, q);
+        sa[4 * i + e] += mulmod_f(x[4 * i + e], a, a * qr, q);
+      }
+    }
+  }
+
+  // the P fold on q limbs
+  if (tens) {
+    const double pq = (double)mont_reduce(A.p_prod[p], qu, qinv), pqq = pq * qr;
+    const uint4 *b1 = reinterpret_cast<const uint4 *>(A.tensor[0] + my);
+    const uint4 *a1 = reinterpret_cast<const uint4 *>(A.tensor[1] + my);
+    const uint4 *b2 = reinterpret_cast<const uint4 *>(A.tensor[2] + my);
+    const uint4 *a2 = reinterpret_cast<const uint4 *>(A.tensor[3] + my);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const uint4 u1 = b1[i], v1 = a1[i], u2 = b2[i], v2 = a2[i];
+      const uint32_t B1[4] = {u1.x, u1.y, u1.z, u1.w}, A1[4] = {v1.x, v1.y, v1.z, v1.w};
+      const uint32_t B2[4] = {u2.x, u2.y, u2.z, u2.w}, A2[4] = {v2.x, v2.y, v2.z, v2.w};
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const double db1 = (double)B1[e], da1 = (double)A1[e];
+        const double db2 = (double)B2[e], da2 = (double)A2[e];
+        const double d0 = mulmod_f(db1, db2, db2 * qr, q);
+        const double d1 = mulmod_f(db1, da2, da2 * qr, q) + mulmod_f(da1, db2, db2 * qr, q);
+        sb[4 * i + e] += mulmod_f(d0, pq, pqq, q);
+        sa[4 * i + e] += mulmod_f(d1, pq, pqq, q);
+      }
+    }
+  } else if (y < np.num_q && (A.add_b || A.add_a)) {
+    const double pp = (double)A.p_prod[p], ppq = pp * qr;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      if (A.add_b) {
+        const uint4 u = reinterpret_cast<const uint4 *>(A.add_b + my)[i];
+        const uint32_t U[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int e = 0; e < 4; e++) sb[4 * i + e] += mulmod_f((double)U[e], pp, ppq, q);
+      }
+      if (A.add_a) {
+        const uint4 u = reinterpret_cast<const uint4 *>(A.add_a + my)[i];
+        const uint32_t U[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int e = 0; e < 4; e++) sa[4 * i + e] += mulmod_f((double)U[e], pp, ppq, q);
+      }
+    }
+  }
+
+  uint4 *ob = reinterpret_cast<uint4 *>(A.out_b + my);
+  uint4 *oa = reinterpret_cast<uint4 *>(A.out_a + my);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    uint32_t vb[4], va[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      vb[e] = canon_small(mulmod_f(sb[4 * i + e], rinv, rinvq, q), q);
+      va[e] = canon_small(mulmod_f(sa[4 * i + e], rinv, rinvq, q), q);
+    }
+    ob[i] = make_uint4(vb[0], vb[1], vb[2], vb[3]);
+    oa[i] = make_uint4(va[0], va[1], va[2], va[3]);
+  }
+}
+
+__global__ __launch_bounds__(kRowThreads, 2) void key_rows_kernel(
+    const EwNp np, const PrimeTab P, const KeyAccumArgs A, const uint32_t *__restrict__ tw) {
+  __shared__ double ldsd[kRowTile * 256];
+  __shared__ uint32_t tws[30 * kRowThreads];
+  key_rows_body(np, P, A, tw, blockIdx.y, ldsd, tws);
+}
+
+__global__ __launch_bounds__(kRowThreads, 2) void key_rows_table_kernel(
+    const KeyAccumItem *__restrict__ items, const EwJob *__restrict__ jobs,
+    const PrimeTab P, const uint32_t *__restrict__ tw) {
+  __shared__ double ldsd[kRowTile * 256];
+  __shared__ uint32_t tws[30 * kRowThreads];
+  typedef const __attribute__((address_space(4))) KeyAccumItem CItem;
+  typedef const __attribute__((address_space(4))) EwJob CJob;
+  CJob &jb = ((CJob *)jobs)[blockIdx.y];
+  const int item = jb.item, y = jb.y;
+  CItem &it = ((CItem *)items)[item];
+  const EwNp np{it.np.num_q, it.np.num_total, it.np.q_prime0, it.np.aux_prime0};
+  key_rows_body(np, P, it.A, tw, y, ldsd, tws);
+}
+
 const char *fwd_cols_name(int flags) {
   return (flags & kNttBconvInput) ? "ntt_fwd_cols+bconv" : "ntt_fwd_cols";
 }
@@ -584,8 +809,9 @@ void fwd_passes(const Context &ctx, const D &d, int n, int flags,
                 hipStream_t st) {
   launch(ctx, fwd_cols_name(flags), n, ntt_fwd_cols<D>, dim3(kColTiles * n),
          dim3(kColThreads), st, d, ctx.ptab, (const uint32_t *)ctx.d_tw_fwd);
-  launch(ctx, fwd_rows_name(flags), n, ntt_fwd_rows<D>, dim3(kRowTiles, n),
-         dim3(kRowThreads), st, d, ctx.ptab, (const uint32_t *)ctx.d_tw_fwd);
+  if (!(flags & kNttColsOnly))
+    launch(ctx, fwd_rows_name(flags), n, ntt_fwd_rows<D>, dim3(kRowTiles, n),
+           dim3(kRowThreads), st, d, ctx.ptab, (const uint32_t *)ctx.d_tw_fwd);
 }
 
 template <class D>
@@ -617,6 +843,20 @@ void launch_ntt_fwd_table(const Context &ctx, const LimbJob *jobs, int n,
                           int flags_any, hipStream_t st) {
   if (n <= 0) return;
   fwd_passes(ctx, JobTable{jobs}, n, flags_any, st);
+}
+
+void launch_key_rows(const Context &ctx, const EwNp &np, const KeyAccumArgs &A,
+                     hipStream_t st) {
+  if (np.num_total <= 0) return;
+  launch(ctx, "ntt_fwd_rows+key", np.num_total, key_rows_kernel, dim3(kRowTiles, np.num_total),
+         dim3(kRowThreads), st, np, ctx.ptab, A, (const uint32_t *)ctx.d_tw_fwd);
+}
+
+void launch_key_rows_table(const Context &ctx, const KeyAccumItem *items, const EwJob *jobs,
+                           int njobs, hipStream_t st) {
+  if (njobs <= 0) return;
+  launch(ctx, "ntt_fwd_rows+key[batch]", njobs, key_rows_table_kernel, dim3(kRowTiles, njobs),
+         dim3(kRowThreads), st, items, jobs, ctx.ptab, (const uint32_t *)ctx.d_tw_fwd);
 }
 
 void launch_intt_table(const Context &ctx, const LimbJob *jobs, int n,

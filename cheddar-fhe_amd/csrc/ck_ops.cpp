@@ -125,6 +125,11 @@ int ImmediateSink::bconv(const std::vector<BconvInst> &insts) {
   return launch_status("bconv");
 }
 
+int ImmediateSink::key_rows(const EwNp &np, const KeyAccumArgs &A) {
+  launch_key_rows(c_, np, A, st_);
+  return launch_status("key rows");
+}
+
 int ImmediateSink::key_accum(const EwNp &np, const KeyAccumArgs &A) {
   launch_key_accum(c_, np, A, st_);
   return launch_status("key accumulate");
@@ -207,7 +212,8 @@ int op_intt(const Context &c, uint32_t *dst, const uint32_t *src, int nm,
 // src (*) prod2 (HMult's a1 (*) a2). copy_own = false leaves each group's own
 // limbs unwritten (the fused inner product reads them from the source).
 int op_modup(const Context &c, int level, uint32_t *dst, const uint32_t *src,
-             uint32_t *v, Sink &s, bool copy_own, const uint32_t *prod2) {
+             uint32_t *v, Sink &s, bool copy_own, const uint32_t *prod2,
+             bool cols_only) {
   const LevelPlan &lp = c.levels[level];
   const int tot = lp.num_q + c.alpha;
   // 1. INTT of all q limbs with the per-group constant, centred output
@@ -269,7 +275,8 @@ int op_modup(const Context &c, int level, uint32_t *dst, const uint32_t *src,
     }
   }
   int rc = mfma ? s.bconv(insts) : kOk;
-  if (!rc) rc = s.pass(true, mfma ? 0 : kNttBconvInput, jobs);
+  if (!rc)
+    rc = s.pass(true, (mfma ? 0 : kNttBconvInput) | (cols_only ? kNttColsOnly : 0), jobs);
   if (rc) return rc;
   if (copy_own) {
     for (int g = 0; g < lp.beta; g++) {
@@ -411,8 +418,9 @@ int op_mult_key_no_moddown(const Context &c, int level, uint32_t *acc_b,
   const LevelPlan &lp = c.levels[level];
   uint32_t *v = ws;
   uint32_t *modup = ws + (size_t)lp.num_q * kLimb;
+  const bool fused = c.fuse_key_rows;
   int rc = op_modup(c, level, modup, tensor ? tensor[1] : x, v, s, false,
-                    tensor ? tensor[3] : nullptr);
+                    tensor ? tensor[3] : nullptr, fused);
   if (rc) return rc;
   KeyAccumArgs A;
   std::memset(&A, 0, sizeof(A));
@@ -428,7 +436,8 @@ int op_mult_key_no_moddown(const Context &c, int level, uint32_t *acc_b,
   A.p_prod = c.d_p_prod;
   A.out_b = acc_b;
   A.out_a = acc_a;
-  return s.key_accum(make_np(c, lp.nm, lp.nt, c.alpha), A);
+  const EwNp np = make_np(c, lp.nm, lp.nt, c.alpha);
+  return fused ? s.key_rows(np, A) : s.key_accum(np, A);
 }
 
 size_t relin_ws_limbs(const Context &c, int level) {

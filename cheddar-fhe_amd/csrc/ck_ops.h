@@ -40,6 +40,8 @@ class Sink {
   // whether ops should emit bconv() steps rather than fused BaseConv passes
   virtual bool mfma_bconv() const = 0;
   virtual int key_accum(const EwNp &np, const KeyAccumArgs &A) = 0;
+  // forward row stages of a column-pass-only ModUp fused with key_accum
+  virtual int key_rows(const EwNp &np, const KeyAccumArgs &A) = 0;
   virtual int bs_fused(const EwNp &np, const BsFusedArgs &A) = 0;
   virtual int permute(const EwNp &np, const EwOperand &x, const EwOperand &out,
                       uint32_t g, int npoly) = 0;
@@ -55,6 +57,7 @@ class ImmediateSink : public Sink {
   int bconv(const std::vector<BconvInst> &insts) override;
   bool mfma_bconv() const override { return c_.bconv_mfma == 1; }
   int key_accum(const EwNp &np, const KeyAccumArgs &A) override;
+  int key_rows(const EwNp &np, const KeyAccumArgs &A) override;
   int bs_fused(const EwNp &np, const BsFusedArgs &A) override;
   int permute(const EwNp &np, const EwOperand &x, const EwOperand &out,
               uint32_t g, int npoly) override;
@@ -71,9 +74,11 @@ int op_intt(const Context &c, uint32_t *dst, const uint32_t *src, int nm,
 int op_intt_const(const Context &c, uint32_t *dst, const uint32_t *src,
                   int nm, int nt, int na, int src_gap,
                   const uint32_t *consts_mont, bool centred, Sink &s);
+// cols_only: the forward transforms of the outputs stop after the column
+// stages (lazy int32 in dst), for a key_rows pass to finish
 int op_modup(const Context &c, int level, uint32_t *dst, const uint32_t *src,
              uint32_t *v, Sink &s, bool copy_own = true,
-             const uint32_t *prod2 = nullptr);
+             const uint32_t *prod2 = nullptr, bool cols_only = false);
 const ModDownPlan *moddown_plan(const Context &c, int level, int kind);
 int op_moddown(const Context &c, int level, int kind, int npoly,
                uint32_t *const *dst, const uint32_t *const *src, uint32_t *v,

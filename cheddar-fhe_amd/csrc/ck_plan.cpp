@@ -87,6 +87,14 @@ int PlanSink::key_accum(const EwNp &np, const KeyAccumArgs &A) {
   return kOk;
 }
 
+int PlanSink::key_rows(const EwNp &np, const KeyAccumArgs &A) {
+  PlanStage &s = stage(PlanStage::kKeyRows);
+  const uint16_t item = (uint16_t)s.ka.size();
+  s.ka.push_back(KeyAccumItem{np, A});
+  for (int y = 0; y < np.num_total; y++) s.ew.push_back(EwJob{item, (uint16_t)y});
+  return kOk;
+}
+
 int PlanSink::bs_fused(const EwNp &np, const BsFusedArgs &A) {
   PlanStage &s = stage(PlanStage::kBsFused);
   const uint16_t item = (uint16_t)s.bs.size();
@@ -117,11 +125,11 @@ void order_by_prime(PlanStage &s) {
   if (!(s.flags_any & kNttBconvInput))
     std::stable_sort(s.jobs.begin(), s.jobs.end(),
                      [](const LimbJob &a, const LimbJob &b) { return a.prime < b.prime; });
+  const bool ka = s.kind == PlanStage::kKeyAccum || s.kind == PlanStage::kKeyRows;
   auto prime_of = [&](const EwJob &j) {
-    return s.kind == PlanStage::kKeyAccum ? ew_prime(s.ka[j.item].np, j.y)
-                                          : ew_prime(s.bs[j.item].np, j.y);
+    return ka ? ew_prime(s.ka[j.item].np, j.y) : ew_prime(s.bs[j.item].np, j.y);
   };
-  if (s.kind == PlanStage::kKeyAccum || s.kind == PlanStage::kBsFused)
+  if (ka || s.kind == PlanStage::kBsFused)
     std::stable_sort(s.ew.begin(), s.ew.end(), [&](const EwJob &a, const EwJob &b) {
       return prime_of(a) < prime_of(b);
     });
@@ -173,6 +181,9 @@ int plan_run(const Plan &p, hipStream_t st) {
         break;
       case PlanStage::kKeyAccum:
         launch_key_accum_table(c, s.d_ka, s.d_ew, (int)s.ew.size(), st);
+        break;
+      case PlanStage::kKeyRows:
+        launch_key_rows_table(c, s.d_ka, s.d_ew, (int)s.ew.size(), st);
         break;
       case PlanStage::kBsFused:
         launch_bs_fused_table(c, s.d_bs, s.d_ew, (int)s.ew.size(), st);
@@ -226,7 +237,10 @@ double plan_time_us(const Plan &p, int reps) {
 int plan_launches(const Plan &p) {
   int n = 0;
   for (const PlanStage &s : p.stages)
-    n += (s.kind == PlanStage::kFwd || s.kind == PlanStage::kInv) ? 2 : 1;
+    n += ((s.kind == PlanStage::kFwd && !(s.flags_any & kNttColsOnly)) ||
+          s.kind == PlanStage::kInv)
+             ? 2
+             : 1;
   return n;
 }
 

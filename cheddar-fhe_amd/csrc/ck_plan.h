@@ -8,7 +8,7 @@ namespace ck {
 // One step of a plan: a transform pass over every job's limb, or one
 // key-accumulation kernel over every item's limbs.
 struct PlanStage {
-  enum Kind { kFwd, kInv, kKeyAccum, kBsFused, kBconv } kind = kFwd;
+  enum Kind { kFwd, kInv, kKeyAccum, kKeyRows, kBsFused, kBconv } kind = kFwd;
   int flags_any = 0;
   std::vector<LimbJob> jobs;
   std::vector<KeyAccumItem> ka;
@@ -46,6 +46,7 @@ class PlanSink : public Sink {
   int copy(uint32_t *dst, const uint32_t *src, size_t words) override;
   int bconv(const std::vector<BconvInst> &insts) override;
   int key_accum(const EwNp &np, const KeyAccumArgs &A) override;
+  int key_rows(const EwNp &np, const KeyAccumArgs &A) override;
   int bs_fused(const EwNp &np, const BsFusedArgs &A) override;
   int permute(const EwNp &np, const EwOperand &x, const EwOperand &out,
               uint32_t g, int npoly) override;

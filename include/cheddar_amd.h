@@ -308,6 +308,12 @@ int ck_plan_destroy(ck_plan *plan);
  * write the plan's outputs) and keep the faster; 0 (default) uses
  * CK_OPT_BCONV_MFMA. */
 #define CK_OPT_PLAN_AUTOTUNE 2
+/* CK_OPT_FUSED_KEY_ROWS: 1 (default) runs the last ModUp transform stages
+ * (forward row stages) and the key inner product of every key switch
+ * (HMult, Relinearize, MultKey, HRot, HConj and their plans) as one pass;
+ * 0 runs them as a row pass plus a separate accumulation. Results are
+ * identical. */
+#define CK_OPT_FUSED_KEY_ROWS 3
 int ck_context_set_option(ck_context *ctx, int option, int value);
 
 /* ---- Per-kernel timing ---------------------------------------------------- */

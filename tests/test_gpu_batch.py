@@ -21,13 +21,18 @@ def ck():
     return cheddar_amd
 
 
-@pytest.fixture(scope="module", params=["bconv-fused", "bconv-mfma", "autotune"])
+@pytest.fixture(scope="module",
+                params=["bconv-fused", "bconv-mfma", "autotune", "key-accum-pass"])
 def ctx24(ck, rns24, request):
     ctx = ck.Context.from_json(rns24)
     if request.param == "autotune":
         ctx.set_plan_autotune(True)
     else:
         ctx.set_bconv_mfma(request.param == "bconv-mfma")
+    # key-accum-pass: the key inner product as its own step, not fused with
+    # the ModUp's forward row stages
+    ctx.fused_key_rows = request.param != "key-accum-pass"
+    ctx.set_fused_key_rows(ctx.fused_key_rows)
     return ctx
 
 
@@ -54,9 +59,11 @@ def test_plan_hmult_mixed_levels(ck, ctx24, oracle_rns24, key24):
     d2 = [[ck.to_device(v) for v in c] for c in cts2]
     plan = ck.Plan.hmult(ctx24, levels, outs, d1, d2, dev_key(*key24))
     info = plan.info()
-    # 9 launches (BaseConv fused) or 11 (two BaseConv steps on the matrix cores)
+    # 9 launches (BaseConv fused) or 11 (two BaseConv steps on the matrix
+    # cores); one fewer with the key product fused into the ModUp's row stages
     assert info["items"] == len(levels)
-    assert info["launches"] == (11 if info["bconv_mfma"] else 9)
+    assert info["launches"] == ((11 if info["bconv_mfma"] else 9)
+                                - (1 if ctx24.fused_key_rows else 0))
     if info["tuned_us"] is not None:  # autotuned: the kept form is the faster
         fused, mfma = info["tuned_us"]
         assert min(fused, mfma) > 0

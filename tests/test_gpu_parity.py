@@ -23,20 +23,27 @@ def ck():
     return cheddar_amd
 
 
-# every test runs with BaseConv fused into the forward NTT column pass and
-# with BaseConv on the FP64 matrix cores: outputs must be identical
-@pytest.fixture(scope="module", params=["bconv-fused", "bconv-mfma"])
+# every test runs with BaseConv fused into the forward NTT column pass, with
+# BaseConv on the FP64 matrix cores, and (BaseConv fused) with the key inner
+# product as its own accumulation pass instead of fused with the ModUp's row
+# stages: outputs must be identical
+CONFIGS = ["bconv-fused", "bconv-mfma", "key-accum-pass"]
+
+
+def configure(ctx, name):
+    ctx.set_bconv_mfma(name == "bconv-mfma")
+    ctx.set_fused_key_rows(name != "key-accum-pass")
+    return ctx
+
+
+@pytest.fixture(scope="module", params=CONFIGS)
 def ctx24(ck, rns24, request):
-    ctx = ck.Context.from_json(rns24)
-    ctx.set_bconv_mfma(request.param == "bconv-mfma")
-    return ctx
+    return configure(ck.Context.from_json(rns24), request.param)
 
 
-@pytest.fixture(scope="module", params=["bconv-fused", "bconv-mfma"])
+@pytest.fixture(scope="module", params=CONFIGS)
 def ctx40(ck, boot40, request):
-    ctx = ck.Context.from_json(boot40)
-    ctx.set_bconv_mfma(request.param == "bconv-mfma")
-    return ctx
+    return configure(ck.Context.from_json(boot40), request.param)
 
 
 def sync():
