@@ -169,7 +169,7 @@ def pmc_traffic(fetch_csv, write_csv):
             grid = int(r["Grid_Size"])
             if short in ("ntt_fwd_cols", "intt_cols"):
                 n = grid // wg // 16
-            elif short in ("ntt_fwd_rows", "intt_rows"):
+            elif short in ("ntt_fwd_rows", "intt_rows", "key_rows"):
                 n = grid // wg // 16
             elif short in ("key_accum", "tensor", "permute"):
                 n = grid // wg // 64
