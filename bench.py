@@ -108,7 +108,7 @@ def _short(sym):
 _KERNELS = ("ntt_fwd_cols", "intt_cols", "ntt_fwd_rows", "intt_rows", "key_accum",
             "key_accum_table", "bs_fused", "bs_fused_table", "tensor", "permute",
             "binary", "unary", "const", "accum", "imag_unit", "crt_lift", "bconv_mfma",
-            "bconv_mfma_table", "key_rows", "key_rows_table")
+            "bconv_mfma_table", "key_rows", "key_rows_table", "gs_accum")
 
 
 def _limbs(short, grid_x, grid_y, wg_x):

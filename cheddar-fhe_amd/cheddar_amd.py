@@ -103,6 +103,9 @@ _SIGS = {
     "ck_plan_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
     "ck_plan_destroy": (ctypes.c_int, [_vp]),
     "ck_context_set_option": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "ck_gs_accum": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                    ctypes.POINTER(_vp), _vp]),
     "ck_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ck_profile_report": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p,
                                          ctypes.POINTER(ctypes.c_double),
@@ -302,6 +305,15 @@ class Context:
         flat = [v for term in xs for v in term]
         _check(lib.ck_paccum(self._h, *np3, len(dsts), len(ys), _ptrs(dsts), _ptrs(flat),
                              _ptrs(ys), int(accumulate), _stream(stream)))
+
+    def gs_accum(self, dsts, xs, pts, np3, stream=None):
+        """dsts[g] = [sum_t xs[t][j] * pts[g][t] for j in (b, a)] for every
+        giant step g; xs: per-baby-step [b, a]; pts[g][t] may be None"""
+        flat_d = [v for pair in dsts for v in pair]
+        flat_x = [v for pair in xs for v in pair]
+        flat_p = [v for row in pts for v in row]
+        _check(lib.ck_gs_accum(self._h, *np3, len(xs), len(dsts), _ptrs(flat_d),
+                               _ptrs(flat_x), _ptrs(flat_p), _stream(stream)))
 
     def caccum(self, dsts, xs, consts, np3, accumulate=False, stream=None):
         flat = [v for term in xs for v in term]

@@ -405,6 +405,34 @@ int ck_caccum(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly
                       accumulate, stream);
 }
 
+int ck_gs_accum(ck_context *ctx, int num_main, int num_ter, int num_aux, int nbs,
+                int ngs, uint32_t *const *dst, const uint32_t *const *x,
+                const uint32_t *const *pt, void *stream) {
+  CK_REQUIRE(ctx && ctx->c && dst && x && pt, "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_np(c, num_main, num_ter, num_aux), "invalid NP");
+  CK_REQUIRE(nbs >= 1 && nbs <= kMaxGsBaby, "nbs must be 1..16");
+  CK_REQUIRE(ngs >= 0, "ngs must not be negative");
+  CK_REQUIRE(valid_ptrs(x, 2 * nbs) && valid_ptrs(dst, 2 * ngs), "null pointer");
+  const EwNp np = make_np(c, num_main, num_ter, num_aux);
+  for (int g0 = 0; g0 < ngs; g0 += kMaxGsGiant) {
+    GsAccumArgs A{};
+    A.nbs = nbs;
+    A.ngs = std::min(ngs - g0, kMaxGsGiant);
+    for (int t = 0; t < nbs; t++) {
+      A.x[t][0] = x[2 * t];
+      A.x[t][1] = x[2 * t + 1];
+    }
+    for (int g = 0; g < A.ngs; g++) {
+      A.dst[g][0] = dst[2 * (g0 + g)];
+      A.dst[g][1] = dst[2 * (g0 + g) + 1];
+      for (int t = 0; t < nbs; t++) A.pt[g][t] = pt[(size_t)(g0 + g) * nbs + t];
+    }
+    launch_gs_accum(c, np, A, as_stream(stream));
+  }
+  return check_hip(hipGetLastError(), "gs_accum");
+}
+
 int ck_tensor(ck_context *ctx, int num_main, int num_ter, uint32_t *d0,
               uint32_t *d1, uint32_t *d2, const uint32_t *b1,
               const uint32_t *a1, const uint32_t *b2, const uint32_t *a2,

@@ -83,6 +83,19 @@ struct BsFusedArgs {
 // per limb. All operands hold num_total contiguous limbs.
 constexpr int kMaxAccTerms = 16;
 
+// Giant-step accumulation of a hoisted linear transform (GSFusedKernel,
+// src/extension/Hoist.cu:80-118): for each of ngs giant steps g,
+//   dst[g][j] = sum_t x[t][j] (*) pt[g][t]   (j = b, a; pt null: no term)
+// with every baby step x[t] read once for all giant steps.
+constexpr int kMaxGsBaby = 16;
+constexpr int kMaxGsGiant = 8;  // per launch
+struct GsAccumArgs {
+  const uint32_t *x[kMaxGsBaby][2];
+  const uint32_t *pt[kMaxGsGiant][kMaxGsBaby];
+  uint32_t *dst[kMaxGsGiant][2];
+  int nbs, ngs;
+};
+
 struct AccumArgs {
   const uint32_t *x[kMaxAccTerms][2];
   const uint32_t *y[kMaxAccTerms];
@@ -131,6 +144,8 @@ void launch_crt_lift(const Context &ctx, const uint32_t *src, const LiftArgs &A,
                      int ndst, uint32_t *dst, hipStream_t st);
 void launch_accum(const Context &ctx, const EwNp &np, const AccumArgs &A,
                   hipStream_t st);
+void launch_gs_accum(const Context &ctx, const EwNp &np, const GsAccumArgs &A,
+                     hipStream_t st);
 void launch_tensor(const Context &ctx, const EwNp &np, const EwOperand &c1,
                    const EwOperand &c2, const EwOperand &out, hipStream_t st);
 void launch_key_accum(const Context &ctx, const EwNp &np,

@@ -165,6 +165,52 @@ __global__ __launch_bounds__(kThreads) void accum_kernel(const EwNp np,
   }
 }
 
+// Giant-step accumulation (GsAccumArgs). NB (>= nbs) bounds the baby steps
+// held in registers, VEC the coefficients per thread (4, or 2 for 16 baby
+// steps). Every load is unconditional (a missing plaintext or a baby step
+// past nbs reads a valid operand and is masked to zero afterwards), so the
+// loads of a giant step are all in flight before its first product.
+template <int NB, int VEC>
+__global__ __launch_bounds__(kThreads) void gs_accum_kernel(const EwNp np,
+                                                            const PrimeTab P,
+                                                            const GsAccumArgs A) {
+  typedef uint32_t vec_t __attribute__((ext_vector_type(VEC)));
+  typedef __attribute__((address_space(1))) vec_t gvec_t;
+  const int y = blockIdx.y;
+  const int k = (blockIdx.x * kThreads + threadIdx.x) * VEC;
+  const int p = ew_prime(np, y);
+  const uint32_t q = P.q[p];
+  const int32_t qinv = P.qinv[p];
+  const size_t off = (size_t)y * 65536 + k;
+  vec_t xb[NB], xa[NB];
+#pragma unroll
+  for (int t = 0; t < NB; t++) {
+    const int tt = t < A.nbs ? t : 0;
+    xb[t] = *(const gvec_t *)(A.x[tt][0] + off);
+    xa[t] = *(const gvec_t *)(A.x[tt][1] + off);
+  }
+  for (int g = 0; g < A.ngs; g++) {
+    vec_t m[NB];
+#pragma unroll
+    for (int t = 0; t < NB; t++) {
+      const uint32_t *pp = A.pt[g][t < A.nbs ? t : 0];
+      const bool use = t < A.nbs && pp != nullptr;
+      const vec_t v = *(const gvec_t *)((use ? pp : A.x[0][0]) + off);
+      m[t] = use ? v : (vec_t)0;
+    }
+    vec_t rb = (vec_t)0, ra = (vec_t)0;
+#pragma unroll
+    for (int t = 0; t < NB; t++)
+#pragma unroll
+      for (int e = 0; e < VEC; e++) {
+        rb[e] = add_mod(rb[e], mont_mul(xb[t][e], m[t][e], q, qinv), q);
+        ra[e] = add_mod(ra[e], mont_mul(xa[t][e], m[t][e], q, qinv), q);
+      }
+    *(gvec_t *)(A.dst[g][0] + off) = rb;
+    *(gvec_t *)(A.dst[g][1] + off) = ra;
+  }
+}
+
 // ModUpToMax1 / ModUpToMax2 (ElementWise.cu:579-623), generalised to any
 // source and destination primes: coefficient-domain limbs of a one- or
 // two-prime base lifted to every destination prime. Two primes: v = s0 q1
@@ -555,6 +601,20 @@ void launch_tensor(const Context &ctx, const EwNp &np, const EwOperand &c1,
   if (np.num_total <= 0) return;
   launch(ctx, "tensor", np.num_total, tensor_kernel, ew_grid(np),
          dim3(kThreads), st, np, ctx.ptab, c1, c2, out);
+}
+
+void launch_gs_accum(const Context &ctx, const EwNp &np, const GsAccumArgs &A,
+                     hipStream_t st) {
+  if (A.ngs <= 0 || np.num_total <= 0) return;
+  if (A.nbs <= 4)
+    launch(ctx, "gs_accum", np.num_total, gs_accum_kernel<4, 4>,
+           dim3(65536 / (kThreads * 4), np.num_total), dim3(kThreads), st, np, ctx.ptab, A);
+  else if (A.nbs <= 8)
+    launch(ctx, "gs_accum", np.num_total, gs_accum_kernel<8, 4>,
+           dim3(65536 / (kThreads * 4), np.num_total), dim3(kThreads), st, np, ctx.ptab, A);
+  else
+    launch(ctx, "gs_accum", np.num_total, gs_accum_kernel<kMaxGsBaby, 2>,
+           dim3(65536 / (kThreads * 2), np.num_total), dim3(kThreads), st, np, ctx.ptab, A);
 }
 
 void launch_key_accum(const Context &ctx, const EwNp &np,

@@ -177,14 +177,23 @@ int ck_crt_lift(ck_context *ctx, const int *src_primes, int n_src,
                 const int *dst_primes, int n_dst, const uint32_t *src, uint32_t *dst,
                 void *stream);
 /* ElementWiseHandler::PAccum (ElementWise.cu:1437-1643, CPAccum kernels
- * :225-366) and the giant-step accumulation GSFusedKernel
- * (src/extension/Hoist.cu:80-118): for p < npoly (1 or 2),
+ * :225-366): for p < npoly (1 or 2),
  *   dst[p] = (accumulate ? dst[p] : 0) + sum_t x[t * npoly + p] (*) y[t]
  * with y[t] a polynomial; every operand holds the NP's limbs contiguously.
  * Any nterms >= 0 (nterms = 0 clears dst unless accumulating). */
 int ck_paccum(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly,
               int nterms, uint32_t *const *dst, const uint32_t *const *x,
               const uint32_t *const *y, int accumulate, void *stream);
+/* HoistHandler::GSFusedPAccum (src/extension/Hoist.cu:329-451, kernel
+ * GSFusedKernel :80-118): the giant-step accumulations of a hoisted linear
+ * transform in one pass, each baby step read once. For g < ngs, j in {0, 1}
+ * (b, a):
+ *   dst[2 g + j] = sum_{t < nbs, pt[g nbs + t] != NULL} x[2 t + j] (*) pt[g nbs + t]
+ * over the limbs of the NP (every operand holds them contiguously);
+ * 1 <= nbs <= 16, any ngs >= 0. */
+int ck_gs_accum(ck_context *ctx, int num_main, int num_ter, int num_aux, int nbs,
+                int ngs, uint32_t *const *dst, const uint32_t *const *x,
+                const uint32_t *const *pt, void *stream);
 /* ElementWiseHandler::CAccum (same kernels, constant multipliers): as
  * ck_paccum with consts[t] a DEVICE array of one value per limb. */
 int ck_caccum(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly,

@@ -248,6 +248,34 @@ def test_paccum_caccum(ck, ctx24, oracle_rns24, accumulate):
             assert_same(ck.to_host(outs[j]), want, f"{kind} poly {j} acc {accumulate}")
 
 
+@pytest.mark.parametrize("nbs,ngs", [(11, 10), (3, 2), (16, 1)])
+def test_gs_accum(ck, ctx24, oracle_rns24, nbs, ngs):
+    """giant-step accumulation of a hoisted transform (GSFusedKernel,
+    src/extension/Hoist.cu:80-118): every giant step's sum of baby step x
+    plaintext products in one pass; missing plaintexts (None) add nothing;
+    more than 8 giant steps take two launches"""
+    P = oracle_rns24
+    pidx = P.np_primes(24, 0, 8)
+    np3 = (24, 0, 8)
+    xs = [[uniform(P, pidx, SEED_BASE + 400 + 2 * t + j) for j in range(2)] for t in range(nbs)]
+    rng = np.random.default_rng(SEED_BASE + 470 + nbs)
+    pts = [[uniform(P, pidx, SEED_BASE + 500 + g * 16 + t) if rng.random() < 0.8 else None
+            for t in range(nbs)] for g in range(ngs)]
+    dx = [[ck.to_device(v) for v in term] for term in xs]
+    dp = [[None if v is None else ck.to_device(v) for v in row] for row in pts]
+    outs = [[ck.to_device(np.full_like(xs[0][0], 0xFFFFFFFF)) for _ in range(2)]
+            for _ in range(ngs)]
+    ctx24.gs_accum(outs, dx, dp, np3)
+    sync()
+    for g in range(ngs):
+        for j in range(2):
+            want = np.zeros_like(xs[0][j])
+            for t in range(nbs):
+                if pts[g][t] is not None:
+                    want = P.add(want, P.mult(xs[t][j], pts[g][t], pidx), pidx)
+            assert_same(ck.to_host(outs[g][j]), want, f"giant step {g} poly {j}")
+
+
 @pytest.mark.parametrize("num_aux", [0, 12])
 def test_boot40_modup_to_max(ck, ctx40, oracle_boot40, num_aux):
     P = oracle_boot40
