@@ -106,6 +106,9 @@ _SIGS = {
     "ck_gs_accum": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                     ctypes.POINTER(_vp), _vp]),
+    "ck_permute_accum": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, ctypes.POINTER(_vp), ctypes.c_int,
+                                         ctypes.POINTER(_vp), _ip, ctypes.c_int, _vp]),
     "ck_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ck_profile_report": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p,
                                          ctypes.POINTER(ctypes.c_double),
@@ -314,6 +317,13 @@ class Context:
         flat_p = [v for row in pts for v in row]
         _check(lib.ck_gs_accum(self._h, *np3, len(xs), len(dsts), _ptrs(flat_d),
                                _ptrs(flat_x), _ptrs(flat_p), _stream(stream)))
+
+    def permute_accum(self, dsts, srcs, rots, np3, accumulate=False, stream=None):
+        """dsts[j] (+)= sum_s sigma_{rots[s]}(srcs[s][j]); srcs[s][j] may be None"""
+        flat = [v for row in srcs for v in row]
+        r = (ctypes.c_int * max(1, len(rots)))(*rots)
+        _check(lib.ck_permute_accum(self._h, *np3, len(dsts), _ptrs(dsts), len(srcs),
+                                    _ptrs(flat), r, int(accumulate), _stream(stream)))
 
     def caccum(self, dsts, xs, consts, np3, accumulate=False, stream=None):
         flat = [v for term in xs for v in term]

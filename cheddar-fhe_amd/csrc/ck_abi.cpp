@@ -433,6 +433,42 @@ int ck_gs_accum(ck_context *ctx, int num_main, int num_ter, int num_aux, int nbs
   return check_hip(hipGetLastError(), "gs_accum");
 }
 
+int ck_permute_accum(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly,
+                     uint32_t *const *dst, int nsrc, const uint32_t *const *src,
+                     const int *rots, int accumulate, void *stream) {
+  CK_REQUIRE(ctx && ctx->c && dst && (nsrc == 0 || (src && rots)), "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_np(c, num_main, num_ter, num_aux), "invalid NP");
+  CK_REQUIRE(npoly >= 1 && npoly <= 2, "npoly must be 1 or 2");
+  CK_REQUIRE(nsrc >= 0, "nsrc must not be negative");
+  CK_REQUIRE(valid_ptrs(dst, npoly), "null pointer");
+  for (int s = 0; s < nsrc * npoly; s++)
+    for (int j = 0; j < npoly; j++)
+      CK_REQUIRE(src[s] == nullptr || src[s] != dst[j],
+                 "PermuteAccum does not support a source aliasing the output");
+  const EwNp np = make_np(c, num_main, num_ter, num_aux);
+  int done = 0;
+  do {
+    PermAccArgs A{};
+    A.npoly = npoly;
+    A.nsrc = std::min(nsrc - done, kMaxPermAcc);
+    A.accumulate = accumulate || done > 0;
+    for (int j = 0; j < npoly; j++) A.dst[j] = dst[j];
+    for (int s = 0; s < A.nsrc; s++) {
+      uint32_t g = 1;
+      if (rots[done + s] != 0) {
+        int rc = ck_galois_factor(ctx, rots[done + s], &g);
+        if (rc) return rc;
+      }
+      A.g[s] = g;
+      for (int j = 0; j < npoly; j++) A.src[s][j] = src[(size_t)(done + s) * npoly + j];
+    }
+    launch_permute_accum(c, np, A, as_stream(stream));
+    done += A.nsrc;
+  } while (done < nsrc);
+  return check_hip(hipGetLastError(), "permute_accum");
+}
+
 int ck_tensor(ck_context *ctx, int num_main, int num_ter, uint32_t *d0,
               uint32_t *d1, uint32_t *d2, const uint32_t *b1,
               const uint32_t *a1, const uint32_t *b2, const uint32_t *a2,

@@ -88,6 +88,17 @@ constexpr int kMaxAccTerms = 16;
 //   dst[g][j] = sum_t x[t][j] (*) pt[g][t]   (j = b, a; pt null: no term)
 // with every baby step x[t] read once for all giant steps.
 constexpr int kMaxGsBaby = 16;
+
+// Sum of automorphisms (PermuteAccum, src/core/ElementWise.cu:394-501):
+//   dst[j] = (accumulate ? dst[j] : 0) + sum_s sigma_{g_s}(src[s][j])
+// for j < npoly (1..2); a null src[s][j] adds nothing; g = 1 is the identity.
+constexpr int kMaxPermAcc = 8;
+struct PermAccArgs {
+  const uint32_t *src[kMaxPermAcc][2];
+  uint32_t g[kMaxPermAcc];
+  uint32_t *dst[2];
+  int nsrc, npoly, accumulate;
+};
 constexpr int kMaxGsGiant = 8;  // per launch
 struct GsAccumArgs {
   const uint32_t *x[kMaxGsBaby][2];
@@ -146,6 +157,8 @@ void launch_accum(const Context &ctx, const EwNp &np, const AccumArgs &A,
                   hipStream_t st);
 void launch_gs_accum(const Context &ctx, const EwNp &np, const GsAccumArgs &A,
                      hipStream_t st);
+void launch_permute_accum(const Context &ctx, const EwNp &np, const PermAccArgs &A,
+                          hipStream_t st);
 void launch_tensor(const Context &ctx, const EwNp &np, const EwOperand &c1,
                    const EwOperand &c2, const EwOperand &out, hipStream_t st);
 void launch_key_accum(const Context &ctx, const EwNp &np,

@@ -522,6 +522,40 @@ __global__ __launch_bounds__(kThreads) void permute_kernel(
   }
 }
 
+// PermAccArgs: every source value is loaded (a null source reads dst[0]
+// and is masked to zero) before the first addition
+__global__ __launch_bounds__(kThreads) void permute_accum_kernel(const EwNp np,
+                                                                 const PrimeTab P,
+                                                                 const PermAccArgs A) {
+  const int y = blockIdx.y;
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t q = P.q[ew_prime(np, y)];
+  const size_t base = (size_t)y * 65536;
+  const uint32_t e = (__brev((uint32_t)k) >> 15) + 1u;
+  uint32_t v[kMaxPermAcc][2];
+#pragma unroll
+  for (int s = 0; s < kMaxPermAcc; s++) {
+    const int ss = s < A.nsrc ? s : 0;
+    const uint32_t g = A.g[ss];
+    const uint32_t idx = g == 1u ? (uint32_t)k : __brev((e * g - 1u) & 0x1FFFFu) >> 15;
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const uint32_t *p = A.src[ss][j < A.npoly ? j : 0];
+      const bool use = s < A.nsrc && j < A.npoly && p != nullptr;
+      const uint32_t w = ld1((use ? p : A.dst[0]) + base + idx);
+      v[s][j] = use ? w : 0u;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    if (j >= A.npoly) break;
+    uint32_t acc = A.accumulate ? ld1(A.dst[j] + base + k) : 0u;
+#pragma unroll
+    for (int s = 0; s < kMaxPermAcc; s++) acc = add_mod(acc, v[s][j], q);
+    st1(A.dst[j] + base + k, acc);
+  }
+}
+
 }  // namespace
 
 static dim3 ew_grid(const EwNp &np) { return dim3(kBlocksPerLimb, np.num_total); }
@@ -631,6 +665,13 @@ void launch_permute(const Context &ctx, const EwNp &np, const EwOperand &x,
   launch(ctx, "permute", np.num_total, permute_kernel,
          dim3(65536 / kThreads, np.num_total), dim3(kThreads), st, np, x, out,
          g, npoly);
+}
+
+void launch_permute_accum(const Context &ctx, const EwNp &np, const PermAccArgs &A,
+                          hipStream_t st) {
+  if (np.num_total <= 0) return;
+  launch(ctx, "permute_accum", np.num_total * A.npoly, permute_accum_kernel,
+         dim3(65536 / kThreads, np.num_total), dim3(kThreads), st, np, ctx.ptab, A);
 }
 
 static int bs_grid(int nlimbs) {

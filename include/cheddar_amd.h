@@ -194,6 +194,16 @@ int ck_paccum(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly
 int ck_gs_accum(ck_context *ctx, int num_main, int num_ter, int num_aux, int nbs,
                 int ngs, uint32_t *const *dst, const uint32_t *const *x,
                 const uint32_t *const *pt, void *stream);
+/* ElementWiseHandler::PermuteAccum (ElementWise.cu:1002-1100, kernels
+ * PermuteAccum / PermuteAccumAdd :394-501): a sum of NTT-domain
+ * automorphisms, for j < npoly (1 or 2),
+ *   dst[j] = (accumulate ? dst[j] : 0) + sum_{s < nsrc} sigma_{rots[s]}(src[s npoly + j])
+ * with sigma_r as in ck_permute (r = 0: identity, r = -1: conjugation); a
+ * NULL src entry adds nothing; no source may alias dst; every operand
+ * holds the NP's limbs contiguously. */
+int ck_permute_accum(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly,
+                     uint32_t *const *dst, int nsrc, const uint32_t *const *src,
+                     const int *rots, int accumulate, void *stream);
 /* ElementWiseHandler::CAccum (same kernels, constant multipliers): as
  * ck_paccum with consts[t] a DEVICE array of one value per limb. */
 int ck_caccum(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly,

@@ -342,6 +342,34 @@ def test_permute(ck, ctx24, oracle_rns24, rot):
     assert_same(ck.to_host(out), want, f"permute {rot}")
 
 
+@pytest.mark.parametrize("accumulate", [False, True])
+def test_permute_accum(ck, ctx24, oracle_rns24, accumulate):
+    """sum of automorphisms over Q*P limbs (PermuteAccum, src/core/
+    ElementWise.cu:394-501): 10 sources (two launches of at most 8) with
+    rotations, conjugation and the identity; a missing second polynomial
+    adds nothing to it"""
+    P = oracle_rns24
+    pidx = P.np_primes(24, 0, 8)
+    np3 = (24, 0, 8)
+    rots = [1, 1234, 0, -1, 32767, 5, 0, 77, 4096, 3]
+    srcs = [[uniform(P, pidx, SEED_BASE + 600 + 2 * s + j) for j in range(2)]
+            for s in range(len(rots))]
+    srcs[2][1] = None
+    srcs[7][1] = None
+    init = [uniform(P, pidx, SEED_BASE + 640 + j) for j in range(2)]
+    d = [[None if v is None else ck.to_device(v) for v in row] for row in srcs]
+    outs = [ck.to_device(v) for v in init]
+    ctx24.permute_accum(outs, d, rots, np3, accumulate=accumulate)
+    sync()
+    for j in range(2):
+        want = init[j] if accumulate else np.zeros_like(init[j])
+        for s, r in enumerate(rots):
+            if srcs[s][j] is not None:
+                t = srcs[s][j] if r == 0 else P.permute(srcs[s][j], P.galois(r))
+                want = P.add(want, t, pidx)
+        assert_same(ck.to_host(outs[j]), want, f"permute_accum poly {j}")
+
+
 # ---------------------------------------------------------------------------
 # Context evaluator
 # ---------------------------------------------------------------------------
