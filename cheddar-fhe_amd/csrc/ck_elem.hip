@@ -522,8 +522,10 @@ __global__ __launch_bounds__(kThreads) void permute_kernel(
   }
 }
 
-// PermAccArgs: every source value is loaded (a null source reads dst[0]
-// and is masked to zero) before the first addition
+// PermAccArgs: the NS (>= nsrc) source values of each polynomial are
+// loaded (a null source reads dst[0] and is masked to zero) before the
+// first addition; one body per source bound keeps dummy loads few
+template <int NS>
 __global__ __launch_bounds__(kThreads) void permute_accum_kernel(const EwNp np,
                                                                  const PrimeTab P,
                                                                  const PermAccArgs A) {
@@ -532,9 +534,9 @@ __global__ __launch_bounds__(kThreads) void permute_accum_kernel(const EwNp np,
   const uint32_t q = P.q[ew_prime(np, y)];
   const size_t base = (size_t)y * 65536;
   const uint32_t e = (__brev((uint32_t)k) >> 15) + 1u;
-  uint32_t v[kMaxPermAcc][2];
+  uint32_t v[NS][2];
 #pragma unroll
-  for (int s = 0; s < kMaxPermAcc; s++) {
+  for (int s = 0; s < NS; s++) {
     const int ss = s < A.nsrc ? s : 0;
     const uint32_t g = A.g[ss];
     const uint32_t idx = g == 1u ? (uint32_t)k : __brev((e * g - 1u) & 0x1FFFFu) >> 15;
@@ -551,7 +553,7 @@ __global__ __launch_bounds__(kThreads) void permute_accum_kernel(const EwNp np,
     if (j >= A.npoly) break;
     uint32_t acc = A.accumulate ? ld1(A.dst[j] + base + k) : 0u;
 #pragma unroll
-    for (int s = 0; s < kMaxPermAcc; s++) acc = add_mod(acc, v[s][j], q);
+    for (int s = 0; s < NS; s++) acc = add_mod(acc, v[s][j], q);
     st1(A.dst[j] + base + k, acc);
   }
 }
@@ -670,8 +672,20 @@ void launch_permute(const Context &ctx, const EwNp &np, const EwOperand &x,
 void launch_permute_accum(const Context &ctx, const EwNp &np, const PermAccArgs &A,
                           hipStream_t st) {
   if (np.num_total <= 0) return;
-  launch(ctx, "permute_accum", np.num_total * A.npoly, permute_accum_kernel,
-         dim3(65536 / kThreads, np.num_total), dim3(kThreads), st, np, ctx.ptab, A);
+  const dim3 grid(65536 / kThreads, np.num_total);
+  const int limbs = np.num_total * A.npoly;
+  if (A.nsrc <= 1)
+    launch(ctx, "permute_accum", limbs, permute_accum_kernel<1>, grid, dim3(kThreads), st,
+           np, ctx.ptab, A);
+  else if (A.nsrc == 2)
+    launch(ctx, "permute_accum", limbs, permute_accum_kernel<2>, grid, dim3(kThreads), st,
+           np, ctx.ptab, A);
+  else if (A.nsrc <= 4)
+    launch(ctx, "permute_accum", limbs, permute_accum_kernel<4>, grid, dim3(kThreads), st,
+           np, ctx.ptab, A);
+  else
+    launch(ctx, "permute_accum", limbs, permute_accum_kernel<kMaxPermAcc>, grid,
+           dim3(kThreads), st, np, ctx.ptab, A);
 }
 
 static int bs_grid(int nlimbs) {
