@@ -225,9 +225,10 @@ uint32_t h_to_mont(uint32_t a, uint32_t q);
 void launch_ntt_fwd(const Context &ctx, const NttLaunch &L, hipStream_t st);
 void launch_intt(const Context &ctx, const NttLaunch &L, hipStream_t st);
 // table form: n jobs in device memory; flags_any = OR of the jobs' flags
-// (names the launch for profiling)
+// (names the launch for profiling); max_bs = the largest BaseConv source
+// count among the jobs (selects the column kernel's instantiation)
 void launch_ntt_fwd_table(const Context &ctx, const LimbJob *jobs, int n,
-                          int flags_any, hipStream_t st);
+                          int flags_any, int max_bs, hipStream_t st);
 void launch_intt_table(const Context &ctx, const LimbJob *jobs, int n,
                        int flags_any, hipStream_t st);
 

@@ -28,6 +28,8 @@
 // int32 in (-0.75 q, 0.75 q) and only final outputs are made canonical.
 // Each thread writes a phase's results back to exactly the LDS words it read
 // for that phase, so only the hand-offs between threads need a barrier.
+#include <algorithm>
+
 #include "ck_elem.h"
 #include "ck_internal.h"
 
@@ -279,7 +281,11 @@ __device__ __forceinline__ void bconv_cols_ms(const int32_t *bsrc, int bs,
   }
 }
 
-// bs is uniform per workgroup: one branch-free body per source bound
+// bs is uniform per workgroup: one branch-free body per source bound.
+// WIDE: the instantiation for launches with more than 9 sources, whose
+// 12- and 13-source bodies keep 16-byte loads at four waves per SIMD (at
+// five they only fit with 8-byte loads)
+template <bool WIDE>
 __device__ __forceinline__ void bconv_cols_to_lds(const int32_t *bsrc, int bs,
                                                   const double *tab, int c0,
                                                   const ModF &m,
@@ -289,8 +295,18 @@ __device__ __forceinline__ void bconv_cols_to_lds(const int32_t *bsrc, int bs,
   switch (bs) {
     case 8: bconv_cols_ms<8, 4>(bsrc, bs, tab, c0, m, lds); return;
     case 9: bconv_cols_ms<9, 4, 5>(bsrc, bs, tab, c0, m, lds); return;
-    case 12: bconv_cols_ms<12, 2>(bsrc, bs, tab, c0, m, lds); return;
-    case 13: bconv_cols_ms<13, 2>(bsrc, bs, tab, c0, m, lds); return;
+    case 12:
+      if (WIDE)
+        bconv_cols_ms<12, 4, 6>(bsrc, bs, tab, c0, m, lds);
+      else
+        bconv_cols_ms<12, 2>(bsrc, bs, tab, c0, m, lds);
+      return;
+    case 13:
+      if (WIDE)
+        bconv_cols_ms<13, 4, 5>(bsrc, bs, tab, c0, m, lds);
+      else
+        bconv_cols_ms<13, 2>(bsrc, bs, tab, c0, m, lds);
+      return;
     default: break;
   }
   if (bs <= 4)
@@ -312,8 +328,8 @@ __device__ __forceinline__ int32_t lazy_i32(double x, const ModF &m) {
 // ---------------------------------------------------------------------------
 // Forward, column stages s = 0..7 -> lazy int32 intermediate
 // ---------------------------------------------------------------------------
-template <class D>
-__global__ __launch_bounds__(kColThreads, 5) void ntt_fwd_cols(
+template <class D, bool WIDE>
+__global__ __launch_bounds__(kColThreads, WIDE ? 4 : 5) void ntt_fwd_cols(
     const D desc, const PrimeTab P, const uint32_t *__restrict__ tw) {
   // int32 BaseConv tile [256][16] aliased with the double exchange tile
   __shared__ double ldsd[256 * kColTile];
@@ -343,7 +359,7 @@ __global__ __launch_bounds__(kColThreads, 5) void ntt_fwd_cols(
   double x[16];
   if (J.flags & kNttBconvInput) {
     // twiddles after the BaseConv, whose loads need the registers
-    bconv_cols_to_lds(J.bsrc, J.bs, J.btab, c0, m, lds);
+    bconv_cols_to_lds<WIDE>(J.bsrc, J.bs, J.btab, c0, m, lds);
     load_twiddles();
     __syncthreads();
 #pragma unroll
@@ -805,10 +821,14 @@ const char *inv_rows_name(int flags) {
 }
 
 template <class D>
-void fwd_passes(const Context &ctx, const D &d, int n, int flags,
+void fwd_passes(const Context &ctx, const D &d, int n, int flags, int max_bs,
                 hipStream_t st) {
-  launch(ctx, fwd_cols_name(flags), n, ntt_fwd_cols<D>, dim3(kColTiles * n),
-         dim3(kColThreads), st, d, ctx.ptab, (const uint32_t *)ctx.d_tw_fwd);
+  if ((flags & kNttBconvInput) && max_bs > 9)
+    launch(ctx, fwd_cols_name(flags), n, ntt_fwd_cols<D, true>, dim3(kColTiles * n),
+           dim3(kColThreads), st, d, ctx.ptab, (const uint32_t *)ctx.d_tw_fwd);
+  else
+    launch(ctx, fwd_cols_name(flags), n, ntt_fwd_cols<D, false>, dim3(kColTiles * n),
+           dim3(kColThreads), st, d, ctx.ptab, (const uint32_t *)ctx.d_tw_fwd);
   if (!(flags & kNttColsOnly))
     launch(ctx, fwd_rows_name(flags), n, ntt_fwd_rows<D>, dim3(kRowTiles, n),
            dim3(kRowThreads), st, d, ctx.ptab, (const uint32_t *)ctx.d_tw_fwd);
@@ -829,7 +849,9 @@ void inv_passes(const Context &ctx, const D &d, int n, int flags,
 // on dst (with the optional ModDown epilogue).
 void launch_ntt_fwd(const Context &ctx, const NttLaunch &L, hipStream_t st) {
   if (L.n <= 0) return;
-  fwd_passes(ctx, JobByValue{L}, L.n, L.flags, st);
+  int max_bs = 0;
+  for (int i = 0; i < 4; i++) max_bs = std::max(max_bs, L.bs[i]);
+  fwd_passes(ctx, JobByValue{L}, L.n, L.flags, max_bs, st);
 }
 
 // Inverse NTT of L.n limbs: row stages into dst, then column stages in
@@ -840,9 +862,9 @@ void launch_intt(const Context &ctx, const NttLaunch &L, hipStream_t st) {
 }
 
 void launch_ntt_fwd_table(const Context &ctx, const LimbJob *jobs, int n,
-                          int flags_any, hipStream_t st) {
+                          int flags_any, int max_bs, hipStream_t st) {
   if (n <= 0) return;
-  fwd_passes(ctx, JobTable{jobs}, n, flags_any, st);
+  fwd_passes(ctx, JobTable{jobs}, n, flags_any, max_bs, st);
 }
 
 void launch_key_rows(const Context &ctx, const EwNp &np, const KeyAccumArgs &A,

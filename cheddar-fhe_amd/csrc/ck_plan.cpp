@@ -63,6 +63,7 @@ int PlanSink::pass(bool forward, int flags, const std::vector<PassJob> &jobs) {
     J.c3 = j.c3;
     J.prime = (uint16_t)j.prime;
     J.flags = flags;
+    s.max_bs = std::max(s.max_bs, j.bs);
     s.jobs.push_back(J);
   }
   return kOk;
@@ -174,7 +175,7 @@ int plan_run(const Plan &p, hipStream_t st) {
   for (const PlanStage &s : p.stages) {
     switch (s.kind) {
       case PlanStage::kFwd:
-        launch_ntt_fwd_table(c, s.d_jobs, (int)s.jobs.size(), s.flags_any, st);
+        launch_ntt_fwd_table(c, s.d_jobs, (int)s.jobs.size(), s.flags_any, s.max_bs, st);
         break;
       case PlanStage::kInv:
         launch_intt_table(c, s.d_jobs, (int)s.jobs.size(), s.flags_any, st);

@@ -10,6 +10,7 @@ namespace ck {
 struct PlanStage {
   enum Kind { kFwd, kInv, kKeyAccum, kKeyRows, kBsFused, kBconv } kind = kFwd;
   int flags_any = 0;
+  int max_bs = 0;  // largest BaseConv source count among the jobs
   std::vector<LimbJob> jobs;
   std::vector<KeyAccumItem> ka;
   std::vector<BsFusedItem> bs;

@@ -17,8 +17,11 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
          "--cuda-device-only", "-c", "-o", os.devnull,
          "-Rpass-analysis=kernel-resource-usage"]
 
-# minimum waves per SIMD for the transform passes (LDS allows five)
+# minimum waves per SIMD for the transform passes (LDS allows five); the
+# column pass's instantiation for BaseConv over more than 9 sources (mangled
+# with template argument Lb1E: WIDE = true) is built for four
 MIN_OCCUPANCY = {"ntt_fwd_cols": 5, "ntt_fwd_rows": 5, "intt_rows": 5, "intt_cols": 5}
+WIDE_OCCUPANCY = 4
 
 
 def resources(src):
@@ -48,4 +51,6 @@ def test_no_spills_and_occupancy(src):
         assert r.get("ScratchSize", 0) == 0, (fn, r)
         for k, occ in MIN_OCCUPANCY.items():
             if k in fn:
+                if k == "ntt_fwd_cols" and "Lb1E" in fn:
+                    occ = WIDE_OCCUPANCY
                 assert r.get("Occupancy", 0) >= occ, (fn, r)
