@@ -166,10 +166,19 @@ __global__ __launch_bounds__(kThreads) void accum_kernel(const EwNp np,
 }
 
 // Giant-step accumulation (GsAccumArgs). NB (>= nbs) bounds the baby steps
-// held in registers, VEC the coefficients per thread (4, or 2 for 16 baby
-// steps). Every load is unconditional (a missing plaintext or a baby step
-// past nbs reads a valid operand and is masked to zero afterwards), so the
-// loads of a giant step are all in flight before its first product.
+// held in registers, VEC the coefficients per thread. Every load is
+// unconditional (a missing plaintext or a baby step past nbs reads a valid
+// operand and is masked to zero afterwards), so the loads of a giant step
+// are all in flight before its first product.
+//
+// Arithmetic in exact FP64 instead of 32-bit Montgomery products (whose
+// multiplies issue at a quarter of the FP64 rate): with the plaintext word
+// split as m = mh 2^14 + ml, the sums Sh = sum x mh (< 16 2^31 2^17 = 2^52)
+// and Sl = sum x ml (< 2^49) over at most 16 terms are exact doubles, and
+//   sum_t x_t (*) m_t = R^-1 (Sh 2^14 + Sl)  mod q
+// is one reduction of Sh 2^14 (a power-of-two scaling, exact), one FP64
+// modular product by R^-1 (its operand below 2^49 + q) and a canonical
+// fix-up per output.
 template <int NB, int VEC>
 __global__ __launch_bounds__(kThreads) void gs_accum_kernel(const EwNp np,
                                                             const PrimeTab P,
@@ -179,8 +188,8 @@ __global__ __launch_bounds__(kThreads) void gs_accum_kernel(const EwNp np,
   const int y = blockIdx.y;
   const int k = (blockIdx.x * kThreads + threadIdx.x) * VEC;
   const int p = ew_prime(np, y);
-  const uint32_t q = P.q[p];
-  const int32_t qinv = P.qinv[p];
+  const double q = P.qd[p], qr = P.qr[p];
+  const double rinv = (double)mont_reduce(1, P.q[p], P.qinv[p]), rinvq = rinv * qr;
   const size_t off = (size_t)y * 65536 + k;
   vec_t xb[NB], xa[NB];
 #pragma unroll
@@ -198,14 +207,30 @@ __global__ __launch_bounds__(kThreads) void gs_accum_kernel(const EwNp np,
       const vec_t v = *(const gvec_t *)((use ? pp : A.x[0][0]) + off);
       m[t] = use ? v : (vec_t)0;
     }
-    vec_t rb = (vec_t)0, ra = (vec_t)0;
+    double hb[VEC], lb[VEC], ha[VEC], la[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; e++) hb[e] = lb[e] = ha[e] = la[e] = 0.0;
 #pragma unroll
     for (int t = 0; t < NB; t++)
 #pragma unroll
       for (int e = 0; e < VEC; e++) {
-        rb[e] = add_mod(rb[e], mont_mul(xb[t][e], m[t][e], q, qinv), q);
-        ra[e] = add_mod(ra[e], mont_mul(xa[t][e], m[t][e], q, qinv), q);
+        const double mh = (double)(m[t][e] >> 14), ml = (double)(m[t][e] & 0x3FFFu);
+        const double vb = (double)xb[t][e], va = (double)xa[t][e];
+        hb[e] = __builtin_fma(vb, mh, hb[e]);
+        lb[e] = __builtin_fma(vb, ml, lb[e]);
+        ha[e] = __builtin_fma(va, mh, ha[e]);
+        la[e] = __builtin_fma(va, ml, la[e]);
       }
+    vec_t rb, ra;
+#pragma unroll
+    for (int e = 0; e < VEC; e++) {
+      const double bigb = hb[e] * 16384.0, biga = ha[e] * 16384.0;  // exact
+      const double kb = __builtin_rint(bigb * qr), ka = __builtin_rint(biga * qr);
+      const double sb = __builtin_fma(-kb, q, bigb) + lb[e];
+      const double sa = __builtin_fma(-ka, q, biga) + la[e];
+      rb[e] = canon_small(mulmod_f(sb, rinv, rinvq, q), q);
+      ra[e] = canon_small(mulmod_f(sa, rinv, rinvq, q), q);
+    }
     *(gvec_t *)(A.dst[g][0] + off) = rb;
     *(gvec_t *)(A.dst[g][1] + off) = ra;
   }
@@ -642,15 +667,16 @@ void launch_tensor(const Context &ctx, const EwNp &np, const EwOperand &c1,
 void launch_gs_accum(const Context &ctx, const EwNp &np, const GsAccumArgs &A,
                      hipStream_t st) {
   if (A.ngs <= 0 || np.num_total <= 0) return;
+  const dim3 grid(65536 / (kThreads * 2), np.num_total);
   if (A.nbs <= 4)
-    launch(ctx, "gs_accum", np.num_total, gs_accum_kernel<4, 4>,
-           dim3(65536 / (kThreads * 4), np.num_total), dim3(kThreads), st, np, ctx.ptab, A);
+    launch(ctx, "gs_accum", np.num_total, gs_accum_kernel<4, 2>, grid, dim3(kThreads), st,
+           np, ctx.ptab, A);
   else if (A.nbs <= 8)
-    launch(ctx, "gs_accum", np.num_total, gs_accum_kernel<8, 4>,
-           dim3(65536 / (kThreads * 4), np.num_total), dim3(kThreads), st, np, ctx.ptab, A);
+    launch(ctx, "gs_accum", np.num_total, gs_accum_kernel<8, 2>, grid, dim3(kThreads), st,
+           np, ctx.ptab, A);
   else
-    launch(ctx, "gs_accum", np.num_total, gs_accum_kernel<kMaxGsBaby, 2>,
-           dim3(65536 / (kThreads * 2), np.num_total), dim3(kThreads), st, np, ctx.ptab, A);
+    launch(ctx, "gs_accum", np.num_total, gs_accum_kernel<kMaxGsBaby, 2>, grid,
+           dim3(kThreads), st, np, ctx.ptab, A);
 }
 
 void launch_key_accum(const Context &ctx, const EwNp &np,
