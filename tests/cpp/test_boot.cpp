@@ -167,6 +167,35 @@ int main(int argc, char **argv) {
   }
 
   try {
+    // a hoisted transform with more baby steps (20) than one giant-step
+    // accumulation pass takes (16) and with missing diagonals
+    const int level = ctx->boot_param_.GetCtSStartLevel();
+    PlainHoistMap map;
+    Random r2(0x4348454444415207ull);
+    for (int g : {0, 20, 40})
+      for (int b = 0; b < 20; b++) {
+        if ((b + g) % 7 == 3) continue;
+        Msg d(slots);
+        r2.SampleUniformComplex(d.data(), slots, -0.1, 0.1);
+        map[g][b] = d;
+      }
+    HoistHandler<word> h(*ctx, map, level, P.GetRescalePrimeProd(level));
+    EvkRequest rq;
+    h.AddRequiredRotations(rq);
+    ui.PrepareRotationKey(rq);
+    Msg m(slots);
+    rng.SampleUniformComplex(m.data(), slots, -1.0, 1.0);
+    Ct ct, out;
+    encrypt(ct, m, level, P.GetRescalePrimeProd(level));
+    h.Evaluate(*ctx, out, ct, ui.GetEvkMap());
+    Report("hoisted transform, 20 baby steps x 3 giant steps", h.PlainEvaluate(m), decrypt(out),
+           1e-4);
+  } catch (const std::exception &e) {
+    std::printf("FAIL hoisted transform: %s\n", e.what());
+    g_fail++;
+  }
+
+  try {
     Msg m(slots);
     rng.SampleUniformComplex(m.data(), slots, -1.0, 1.0);
     Ct ct, out;
