@@ -11,17 +11,19 @@
 // MI355X mapping. A limb of 2^16 u32 is a 256 x 256 matrix (row = 256
 // consecutive coefficients).
 //   * row kernels (forward stages 8..15, inverse stages 0..7): a 256-thread
-//     workgroup owns 16 consecutive rows = one contiguous 16 KiB block,
-//     moved with 16-byte loads/stores that cover whole cache lines.
+//     workgroup owns 16 consecutive rows = one contiguous 16 KiB block; the
+//     side a thread holds 16 consecutive words of moves with 16-byte
+//     accesses, the other side straight between registers and memory with
+//     4-byte accesses (64 contiguous bytes per row and instruction).
 //   * column kernels (forward stages 0..7, inverse 8..15): a 256-thread
 //     workgroup owns all 256 rows x 16 columns; every load and store is a
 //     64-byte row segment per 16 lanes, read and written straight from
 //     registers (32 KiB of LDS: four workgroups and more fit a CU).
-// Each thread runs two radix-16 register phases on 16 values. Row tiles
-// move through an int32 LDS staging tile with 16-byte accesses; the hand-off
-// between the two phases keeps the lazy doubles in an LDS tile aliased with
-// the staging tile (row tiles use an XOR swizzle so both phase layouts are
-// free of bank conflicts). Twiddles are read once per workgroup phase as u32
+// Each thread runs two radix-16 register phases on 16 values; the hand-off
+// between the two phases keeps the lazy doubles in an LDS tile (row tiles
+// use an XOR swizzle so both phase layouts are free of bank conflicts; the
+// column pass's BaseConv stages its int32 output in the same LDS words
+// first). Twiddles are read once per workgroup phase as u32
 // and issued before the data loads. Butterflies use the exact FP64 modular
 // product of ck_arith.h with lazy ranges (integer doubles below 2^40); the
 // intermediate between the two passes of a transform is stored as a lazy
@@ -171,32 +173,22 @@ struct JobTable {
   }
 };
 
-// 16 KiB contiguous row tile <-> LDS, 16-byte accesses
-__device__ __forceinline__ void tile_rows_to_lds(const uint32_t *g, int32_t *lds) {
+// Row tile element b = t16 + 16 k of row rl straight from / to global
+// memory: each load or store instruction covers 64 contiguous bytes of each
+// of the wave's four rows, and no LDS staging or barrier is needed.
+__device__ __forceinline__ void rows_load_p1(const uint32_t *tile, int rl, int t16,
+                                             double x[16]) {
+  typedef const __attribute__((address_space(1))) int32_t gint;
+  gint *g = (gint *)(tile + rl * 256 + t16);
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int e = 1024 * i + 4 * threadIdx.x;
-    const uint4 v = *reinterpret_cast<const uint4 *>(g + e);
-    const int rl = e >> 8, b = e & 255;
-    lds[rlds(rl, b)] = (int32_t)v.x;
-    lds[rlds(rl, b + 1)] = (int32_t)v.y;
-    lds[rlds(rl, b + 2)] = (int32_t)v.z;
-    lds[rlds(rl, b + 3)] = (int32_t)v.w;
-  }
+  for (int k = 0; k < 16; k++) x[k] = (double)g[16 * k];
 }
-
-__device__ __forceinline__ void lds_to_tile_rows(const int32_t *lds, uint32_t *g) {
+__device__ __forceinline__ void rows_store_p1(uint32_t *tile, int rl, int t16,
+                                              const int32_t v[16]) {
+  typedef __attribute__((address_space(1))) int32_t gint;
+  gint *g = (gint *)(tile + rl * 256 + t16);
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int e = 1024 * i + 4 * threadIdx.x;
-    const int rl = e >> 8, b = e & 255;
-    uint4 v;
-    v.x = (uint32_t)lds[rlds(rl, b)];
-    v.y = (uint32_t)lds[rlds(rl, b + 1)];
-    v.z = (uint32_t)lds[rlds(rl, b + 2)];
-    v.w = (uint32_t)lds[rlds(rl, b + 3)];
-    *reinterpret_cast<uint4 *>(g + e) = v;
-  }
+  for (int k = 0; k < 16; k++) g[16 * k] = v[k];
 }
 
 // Column tile of a launch: 16 tiles of 16 columns per limb; workgroup i
@@ -399,7 +391,6 @@ template <class D>
 __global__ __launch_bounds__(kRowThreads, 5) void ntt_fwd_rows(
     const D desc, const PrimeTab P, const uint32_t *__restrict__ tw) {
   __shared__ double ldsd[kRowTile * 256];
-  int32_t *lds = reinterpret_cast<int32_t *>(ldsd);
   const int y = blockIdx.y;
   const LimbJob J = desc.job(y);
   const double q = P.qd[J.prime], qr = P.qr[J.prime];
@@ -422,14 +413,9 @@ __global__ __launch_bounds__(kRowThreads, 5) void ntt_fwd_rows(
           w[(4096 << s) + (r << (4 + s)) + (t16 << s) + blk];
     }
   // the pass-1 intermediate (in dst) is lazy signed int32
-  tile_rows_to_lds(J.dst + tile, lds);
-  __syncthreads();
-
   double x[16];
-#pragma unroll
-  for (int k = 0; k < 16; k++) x[k] = (double)lds[rlds(rl, t16 + 16 * k)];
+  rows_load_p1(J.dst + tile, rl, t16, x);
   ct16(x, t1, q, qr);
-  __syncthreads();
 #pragma unroll
   for (int k = 0; k < 16; k++) ldsd[rlds(rl, t16 + 16 * k)] = x[k];
   __syncthreads();
@@ -483,7 +469,6 @@ template <class D>
 __global__ __launch_bounds__(kRowThreads, 5) void intt_rows(
     const D desc, const PrimeTab P, const uint32_t *__restrict__ tw) {
   __shared__ double ldsd[kRowTile * 256];
-  int32_t *lds = reinterpret_cast<int32_t *>(ldsd);
   const int y = blockIdx.y;
   const LimbJob J = desc.job(y);
   const double q = P.qd[J.prime], qr = P.qr[J.prime];
@@ -545,11 +530,10 @@ __global__ __launch_bounds__(kRowThreads, 5) void intt_rows(
 #pragma unroll
   for (int k = 0; k < 16; k++) x[k] = ldsd[rlds(rl, t16 + 16 * k)];
   gs16(x, t2, q, qr);
-  __syncthreads();  // int32 words alias doubles other threads still read
+  int32_t o[16];
 #pragma unroll
-  for (int k = 0; k < 16; k++) lds[rlds(rl, t16 + 16 * k)] = lazy_i32(x[k], m);
-  __syncthreads();
-  lds_to_tile_rows(lds, J.dst + tile);
+  for (int k = 0; k < 16; k++) o[k] = lazy_i32(x[k], m);
+  rows_store_p1(J.dst + tile, rl, t16, o);
 }
 
 // ---------------------------------------------------------------------------
@@ -633,7 +617,6 @@ template <class Args>
 __device__ __forceinline__ void key_rows_body(const EwNp &np, const PrimeTab &P,
                                               Args &A, const uint32_t *tw, int y,
                                               double *ldsd, uint32_t *tws) {
-  int32_t *lds = reinterpret_cast<int32_t *>(ldsd);
   const int p = y < np.num_q ? np.q_prime0 + y : np.aux_prime0 + (y - np.num_q);
   const double q = P.qd[p], qr = P.qr[p];
   const uint32_t qu = P.q[p];
@@ -706,13 +689,9 @@ __device__ __forceinline__ void key_rows_body(const EwNp &np, const PrimeTab &P,
         }
       }
     } else {
-      __syncthreads();  // the previous group's tile may still be read
-      tile_rows_to_lds(A.modup + (size_t)g * tot_stride + (size_t)y * kN + tile, lds);
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < 16; k++) x[k] = (double)lds[rlds(rl, t16 + 16 * k)];
+      rows_load_p1(A.modup + (size_t)g * tot_stride + (size_t)y * kN + tile, rl, t16, x);
       ct16_lds(x, tws, q, qr);
-      __syncthreads();
+      __syncthreads();  // the previous group's exchange may still be read
 #pragma unroll
       for (int k = 0; k < 16; k++) ldsd[rlds(rl, t16 + 16 * k)] = x[k];
       __syncthreads();
