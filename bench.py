@@ -296,6 +296,17 @@ def bench_batched(a, torch, ck):
     hrot_s = BATCH * BATCH_ROTS / (t_rot * 1e-3)
     log(f"batched HMult: {t_mult:.3f} ms per {BATCH} ({hmult_s:.0f}/s); hoisted HRot: "
         f"{t_rot:.3f} ms per {BATCH * BATCH_ROTS} ({hrot_s:.0f}/s)")
+    # per-kernel durations of one run of each plan (library dispatch events)
+    for name, plan in (("hmult", p_mult), ("hoisted", p_rot)):
+        ctx.profile(True)
+        plan.run(stream)
+        torch.cuda.synchronize()
+        rep = ctx.profile_report()
+        ctx.profile(False)
+        tot = sum(v[0] for v in rep.values())
+        for k, (ms, calls) in sorted(rep.items(), key=lambda kv: -kv[1][0]):
+            log(f"  {name} plan: {k:32s} {calls} x {1e3 * ms / calls:8.1f} us "
+                f"({100 * ms / tot:4.1f}%)")
     res = {
         "metric": "hmult_relin_rescale_per_s", "value": hmult_s, "unit": "HMult/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
