@@ -414,6 +414,13 @@ int ck_gs_accum(ck_context *ctx, int num_main, int num_ter, int num_aux, int nbs
   CK_REQUIRE(nbs >= 1 && nbs <= kMaxGsBaby, "nbs must be 1..16");
   CK_REQUIRE(ngs >= 0, "ngs must not be negative");
   CK_REQUIRE(valid_ptrs(x, 2 * nbs) && valid_ptrs(dst, 2 * ngs), "null pointer");
+  // an output aliasing an operand would be read after it was written
+  for (int d = 0; d < 2 * ngs; d++) {
+    for (int t = 0; t < 2 * nbs; t++)
+      CK_REQUIRE(dst[d] != x[t], "gs_accum: an output aliases a baby step");
+    for (int t = 0; t < ngs * nbs; t++)
+      CK_REQUIRE(dst[d] != pt[t], "gs_accum: an output aliases a plaintext");
+  }
   const EwNp np = make_np(c, num_main, num_ter, num_aux);
   for (int g0 = 0; g0 < ngs; g0 += kMaxGsGiant) {
     GsAccumArgs A{};

@@ -190,7 +190,7 @@ int ck_paccum(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly
  * (b, a):
  *   dst[2 g + j] = sum_{t < nbs, pt[g nbs + t] != NULL} x[2 t + j] (*) pt[g nbs + t]
  * over the limbs of the NP (every operand holds them contiguously);
- * 1 <= nbs <= 16, any ngs >= 0. */
+ * 1 <= nbs <= 16, any ngs >= 0; no output may alias an input. */
 int ck_gs_accum(ck_context *ctx, int num_main, int num_ter, int num_aux, int nbs,
                 int ngs, uint32_t *const *dst, const uint32_t *const *x,
                 const uint32_t *const *pt, void *stream);

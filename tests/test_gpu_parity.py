@@ -274,6 +274,9 @@ def test_gs_accum(ck, ctx24, oracle_rns24, nbs, ngs):
                 if pts[g][t] is not None:
                     want = P.add(want, P.mult(xs[t][j], pts[g][t], pidx), pidx)
             assert_same(ck.to_host(outs[g][j]), want, f"giant step {g} poly {j}")
+    # an output aliasing an operand is rejected before anything is launched
+    with pytest.raises(ck.CheddarError):
+        ctx24.gs_accum([[dx[0][0], outs[0][1]]] + outs[1:], dx, dp, np3)
 
 
 @pytest.mark.parametrize("num_aux", [0, 12])
