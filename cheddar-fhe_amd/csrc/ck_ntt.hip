@@ -649,6 +649,26 @@ __device__ __forceinline__ void key_rows_body(const EwNp &np, const PrimeTab &P,
   double sb[16], sa[16];
 #pragma unroll
   for (int j = 0; j < 16; j++) sb[j] = sa[j] = 0.0;
+  // the column-pass tile of the next group that needs the row stages is
+  // loaded one group ahead (into `pre`), so its latency overlaps the
+  // current group's arithmetic
+  auto next_rows_group = [&](int from) {
+    for (int i = from; i < A.ngroups; i++) {
+      const int g = A.group[i];
+      if (!(y >= A.own0[g] && y < A.own1[g])) return i;
+    }
+    return -1;
+  };
+  typedef const __attribute__((address_space(1))) int32_t gint;
+  int32_t pre[16];
+  auto prefetch = [&](int gi) {
+    gint *src = (gint *)(A.modup + (size_t)A.group[gi] * tot_stride + (size_t)y * kN + tile +
+                         rl * 256 + t16);
+#pragma unroll
+    for (int k = 0; k < 16; k++) pre[k] = src[16 * k];
+  };
+  int pg = next_rows_group(0);
+  if (pg >= 0) prefetch(pg);
   for (int gi = 0; gi < A.ngroups; gi++) {
     const int g = A.group[gi];
     const bool own = (y >= A.own0[g] && y < A.own1[g]);
@@ -689,7 +709,11 @@ __device__ __forceinline__ void key_rows_body(const EwNp &np, const PrimeTab &P,
         }
       }
     } else {
-      rows_load_p1(A.modup + (size_t)g * tot_stride + (size_t)y * kN + tile, rl, t16, x);
+      // gi == pg: its tile is in `pre`; start the next one
+#pragma unroll
+      for (int k = 0; k < 16; k++) x[k] = (double)pre[k];
+      pg = next_rows_group(gi + 1);
+      if (pg >= 0) prefetch(pg);
       ct16_lds(x, tws, q, qr);
       __syncthreads();  // the previous group's exchange may still be read
 #pragma unroll
