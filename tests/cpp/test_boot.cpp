@@ -281,7 +281,9 @@ int main(int argc, char **argv) {
     };
     for (const Case &c : {Case{slots / 2, BootVariant::kNormal, "Boot (2^14 slots)"},
                           Case{slots / 4, BootVariant::kImaginaryRemoving,
-                               "Boot imaginary-removing (2^13 slots)"}}) {
+                               "Boot imaginary-removing (2^13 slots)"},
+                          Case{slots / 8, BootVariant::kMergeTwoReal,
+                               "Boot merge-two-real (2^12 slots), x + conj(x)"}}) {
       ctx->PrepareEvalSpecialFFT(c.slots, c.variant);
       EvkRequest r2;
       ctx->AddRequiredRotations(r2, c.slots);
@@ -291,10 +293,12 @@ int main(int argc, char **argv) {
       Ct ct, out;
       encrypt(ct, m, 0, P.GetScale(0));
       ctx->Boot(out, ct, ui.GetEvkMap());
-      Msg want = m;
-      if (c.variant == BootVariant::kImaginaryRemoving)
+      Msg want = m, got = decrypt(out);
+      if (c.variant != BootVariant::kNormal)
         for (auto &v : want) v = Complex(v.real(), 0);
-      Report(c.name, want, decrypt(out), 1e-3);
+      if (c.variant == BootVariant::kMergeTwoReal)
+        for (auto &v : got) v = Complex(2 * v.real(), 0);  // x + conj(x)
+      Report(c.name, want, got, 1e-3);
     }
   } catch (const std::exception &e) {
     std::printf("FAIL sparse-slot bootstrap: %s\n", e.what());
