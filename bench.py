@@ -25,6 +25,7 @@ if PKG not in sys.path:
     sys.path.insert(0, PKG)
 
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
 LEVEL = 23
 PARAMS = "rns24_aux8.json"
 PROFILE_STEPS = 20
@@ -441,6 +442,7 @@ def main():
         cpu = cpu_baseline(prm)
         log(f"cpu baseline: {cpu['value']:.3f} HMult/s")
 
+    config3 = bench_config3(torch, ck, ctx)
     boot = None if a.no_boot else bench_boot(a.boot_runs)
 
     res = {
@@ -464,9 +466,69 @@ def main():
                      "traffic": traffic, "launch_us": launch_us,
                      "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
+        "config3": config3,
         "bootstrap": boot,
     }
     print(json.dumps(res), flush=True)
+
+
+def bench_config3(torch, ck, ctx, reps=10):
+    """The config-3 contractions at their stated shape, 64 polynomials x 32
+    limbs, each as one batched plan: BaseConv 8 -> 32 limbs (the special
+    primes to all primes) on the FP64 matrix cores, and the forward NTT of
+    all 32 limbs of each polynomial. Inputs resident in HBM; HIP events on
+    the launch stream."""
+    npoly, src_p, dst_p = 64, list(range(24, 32)), list(range(32))
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0x4348454444415203)
+    srcs = []
+    for _ in range(npoly):
+        v = torch.empty((len(src_p), ctx.n), dtype=torch.int32, device="cuda")
+        for i, pi in enumerate(src_p):
+            h = (ctx.prime(pi) - 1) // 2
+            v[i] = torch.randint(-h, h + 1, (ctx.n,), generator=gen, device="cuda",
+                                 dtype=torch.int64).to(torch.int32)
+        srcs.append(v)
+    dsts = [torch.empty((len(dst_p), ctx.n), dtype=torch.int32, device="cuda")
+            for _ in range(npoly)]
+    plan = ck.Plan.bconv(ctx, src_p, dst_p, srcs, dsts)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps  # ms
+
+    ms_b = timed(lambda: plan.run(stream))
+    b_bytes = npoly * (len(src_p) + len(dst_p)) * ctx.n * 4
+    b_flop = npoly * len(src_p) * len(dst_p) * ctx.n * 2 * 2   # {hi, lo} FP64 MACs
+    polys = [uniform_poly(torch, ctx, dst_p, gen) for _ in range(npoly)]
+    outs = [torch.empty_like(x) for x in polys]
+    nplan = ck.Plan.ntt(ctx, polys, outs, (ctx.L, 0, ctx.alpha))
+    ms_n = timed(lambda: nplan.run(stream))
+    n_bytes = npoly * len(dst_p) * ctx.n * 8
+    res = {
+        "shape": "64 polynomials x 32 limbs, N=2^16",
+        "bconv_8_to_32_ms": ms_b,
+        "bconv_hbm_gb_s": b_bytes / (ms_b * 1e-3) / 1e9,
+        "bconv_hbm_frac": b_bytes / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "bconv_fp64_mfma_tflop_s": b_flop / (ms_b * 1e-3) / 1e12,
+        "bconv_fp64_mfma_frac": b_flop / (ms_b * 1e-3) / 1e12 / FP64_MFMA_PEAK_TFLOPS,
+        "ntt_fwd_ms": ms_n,
+        "ntt_us_per_limb": 1e3 * ms_n / (npoly * len(dst_p)),
+        "ntt_hbm_gb_s": n_bytes / (ms_n * 1e-3) / 1e9,
+        "ntt_hbm_frac": n_bytes / (ms_n * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
+    log(f"config 3: BaseConv 8->32 x 64 polys {ms_b:.3f} ms ({res['bconv_hbm_gb_s']:.0f} GB/s, "
+        f"{res['bconv_fp64_mfma_tflop_s']:.1f} FP64 MFMA TFLOP/s); forward NTT 64 x 32 limbs "
+        f"{ms_n:.3f} ms ({res['ntt_hbm_gb_s']:.0f} GB/s)")
+    return res
 
 
 def bench_boot(runs):

@@ -109,6 +109,12 @@ _SIGS = {
     "ck_permute_accum": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int, ctypes.POINTER(_vp), ctypes.c_int,
                                          ctypes.POINTER(_vp), _ip, ctypes.c_int, _vp]),
+    "ck_plan_bconv": (ctypes.c_int, [_vp, _ip, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                      ctypes.POINTER(_vp)]),
+    "ck_plan_ntt": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp),
+                                    ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "ck_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ck_profile_report": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p,
                                          ctypes.POINTER(ctypes.c_double),
@@ -461,6 +467,25 @@ class Plan:
                                         _ptrs([o[0] for o in flat]), _ptrs([o[1] for o in flat]),
                                         int(moddown), ctypes.byref(h)))
         return cls(ctx, h, (cts, keys, outs))
+
+    @classmethod
+    def bconv(cls, ctx, src_primes, dst_primes, srcs, dsts):
+        """dsts[k] = BaseConv of the centred int32 limbs srcs[k] from
+        src_primes to dst_primes (ck_plan_bconv)"""
+        n = len(srcs)
+        h = ctypes.c_void_p()
+        _check(lib.ck_plan_bconv(ctx._h, (ctypes.c_int * len(src_primes))(*src_primes),
+                                 len(src_primes), (ctypes.c_int * len(dst_primes))(*dst_primes),
+                                 len(dst_primes), n, _ptrs(srcs), _ptrs(dsts), ctypes.byref(h)))
+        return cls(ctx, h, (srcs, dsts))
+
+    @classmethod
+    def ntt(cls, ctx, srcs, dsts, np3, inverse=False, mont_conv=False):
+        """dsts[k] = NTT (or INTT) of srcs[k] over the limbs of np3 (ck_plan_ntt)"""
+        h = ctypes.c_void_p()
+        _check(lib.ck_plan_ntt(ctx._h, len(srcs), *np3, int(inverse), int(mont_conv),
+                               _ptrs(srcs), _ptrs(dsts), ctypes.byref(h)))
+        return cls(ctx, h, (srcs, dsts))
 
     def run(self, stream=None):
         _check(lib.ck_plan_run(self._h, _stream(stream)))

@@ -3,6 +3,7 @@
 // dispatch to the host orchestration in ck_ops.cpp.
 #include "../../include/cheddar_amd.h"
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -729,6 +730,93 @@ int ck_plan_hmult(ck_context *ctx, int n, const int *levels, int rescale,
   Plan *p = nullptr;
   const int rc = build_tuned(c, build, &p);
   if (rc) return rc;
+  *out = new ck_plan{p};
+  return CK_OK;
+}
+
+int ck_plan_bconv(ck_context *ctx, const int *src_primes, int n_src,
+                  const int *dst_primes, int n_dst, int n, const int32_t *const *src,
+                  uint32_t *const *dst, ck_plan **out) {
+  CK_REQUIRE(out, "out is null");
+  *out = nullptr;
+  CK_REQUIRE(ctx && ctx->c && src_primes && dst_primes && src && dst, "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(n >= 1, "a plan needs at least one polynomial");
+  CK_REQUIRE(n_src >= 1 && n_src <= kMaxBconvSrc, "n_src must be 1..16");
+  CK_REQUIRE(n_dst >= 1 && n_dst <= kMaxMfmaDst, "n_dst must be 1..64");
+  for (int i = 0; i < n_src; i++) {
+    CK_REQUIRE(src_primes[i] >= 0 && src_primes[i] < c.num_primes, "invalid source prime");
+    for (int j = 0; j < i; j++)
+      CK_REQUIRE(src_primes[j] != src_primes[i], "source primes must be distinct");
+  }
+  for (int m = 0; m < n_dst; m++)
+    CK_REQUIRE(dst_primes[m] >= 0 && dst_primes[m] < c.num_primes,
+               "invalid destination prime");
+  for (int k = 0; k < n; k++) CK_REQUIRE(src[k] && dst[k], "null pointer");
+  std::vector<int> sp(src_primes, src_primes + n_src), dp(dst_primes, dst_primes + n_dst);
+  std::vector<uint32_t> c1;
+  std::vector<int32_t> table;
+  std::vector<double> ftab;
+  bconv_constants(c, sp, dp, {}, c1, table, ftab);
+  Plan *p = new Plan;
+  p->ctx = &c;
+  p->items = n;
+  p->bconv_mfma = true;
+  double *d_tab = nullptr;
+  int rc = plan_upload(*p, &d_tab, ftab);
+  PlanSink sink(*p);
+  for (int k = 0; k < n && !rc; k++) {
+    sink.begin_item();
+    BconvInst I;
+    std::memset(&I, 0, sizeof(I));
+    I.src = src[k];
+    I.dst = dst[k];
+    I.tab = d_tab;
+    I.S = n_src;
+    I.D = n_dst;
+    I.canonical = 1;
+    for (int m = 0; m < n_dst; m++) {
+      I.dst_limb[m] = (uint16_t)m;
+      I.dst_prime[m] = (uint8_t)dp[m];
+    }
+    rc = sink.bconv({I});
+  }
+  if (!rc) rc = plan_finish(*p, sink.mismatch());
+  if (rc) {
+    plan_destroy(p);
+    return rc;
+  }
+  *out = new ck_plan{p};
+  return CK_OK;
+}
+
+int ck_plan_ntt(ck_context *ctx, int n, int num_main, int num_ter, int num_aux,
+                int inverse, int montgomery_conversion, const uint32_t *const *src,
+                uint32_t *const *dst, ck_plan **out) {
+  CK_REQUIRE(out, "out is null");
+  *out = nullptr;
+  CK_REQUIRE(ctx && ctx->c && src && dst, "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(n >= 1, "a plan needs at least one polynomial");
+  CK_REQUIRE(valid_np(c, num_main, num_ter, num_aux), "invalid NP");
+  for (int k = 0; k < n; k++) CK_REQUIRE(src[k] && dst[k], "null pointer");
+  Plan *p = new Plan;
+  p->ctx = &c;
+  p->items = n;
+  PlanSink sink(*p);
+  int rc = kOk;
+  for (int k = 0; k < n && !rc; k++) {
+    sink.begin_item();
+    rc = inverse ? op_intt(c, dst[k], src[k], num_main, num_ter, num_aux, 0,
+                           montgomery_conversion != 0, sink)
+                 : op_ntt(c, dst[k], src[k], num_main, num_ter, num_aux, 0,
+                          montgomery_conversion != 0, sink);
+  }
+  if (!rc) rc = plan_finish(*p, sink.mismatch());
+  if (rc) {
+    plan_destroy(p);
+    return rc;
+  }
   *out = new ck_plan{p};
   return CK_OK;
 }

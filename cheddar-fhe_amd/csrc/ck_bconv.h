@@ -18,6 +18,7 @@ struct BconvInst {
   uint32_t *dst;
   const double *tab;
   int S, D;
+  int canonical;  // 1: outputs in [0, t); 0: lazy int32 for a transform pass
   uint16_t dst_limb[kMaxMfmaDst];
   uint8_t dst_prime[kMaxMfmaDst];
 };

@@ -18,7 +18,7 @@
 // at one position and needs no cross-lane exchange.
 //
 // Output: lazy int32 in (-0.75 t, 0.75 t) per destination limb, the input
-// the forward column pass reads.
+// the forward column pass reads, or canonical residues (standalone BaseConv).
 #include "ck_bconv.h"
 
 #include <algorithm>
@@ -115,7 +115,8 @@ __device__ __forceinline__ void bconv_body(Inst &I, const PrimeTab &P, int chunk
           const ModF mf{eq[rt][e], eqr[rt][e]};
           const double big = acc[2 * e] * 65536.0;  // exact (power-of-two scale)
           const double k = __builtin_rint(big * mf.qinv);
-          const double r = reduce_f(__builtin_fma(-k, mf.q, big) + acc[2 * e + 1], mf);
+          double r = reduce_f(__builtin_fma(-k, mf.q, big) + acc[2 * e + 1], mf);
+          if (I.canonical) r = r < 0.0 ? r + mf.q : r;
           edst[rt][e][pos] = (uint32_t)(int32_t)r;
         }
       }

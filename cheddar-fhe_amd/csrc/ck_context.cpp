@@ -121,6 +121,8 @@ int upload(Context &c, T **dptr, const std::vector<T> &h) {
   return rc;
 }
 
+}  // namespace
+
 // PopulateModSwitchConstants (ModSwitch.cu:320-366), kFuseModDownEpilogue:
 // c1[i] = ((prod_{j!=i} s_j) * N)^-1 * prod(restore) mod s_i, returned in
 // plain form (times R^-1 because the reference applies it with a Montgomery
@@ -161,6 +163,8 @@ void bconv_constants(const Context &c, const std::vector<int> &src,
     }
   }
 }
+
+namespace {
 
 // One ModDown flavour at one level (ModSwitch.cu:233-303 for the prime
 // ranges, :566-684 for which limbs are transformed, NTT.cu:456-601 for the

@@ -221,6 +221,16 @@ uint32_t h_powmod(uint32_t a, uint64_t e, uint32_t q);
 uint32_t h_invmod(uint32_t a, uint32_t q);
 uint32_t h_to_mont(uint32_t a, uint32_t q);
 
+// BaseConv constants from `src` to `dst` (global prime indices), see
+// ck_context.cpp: c1 = the INTT post-constant per source, table = the
+// reference's centred (prod_{j!=i} s_j) R^2, ftab = (prod_{j!=i} s_j) R split
+// into {hi, lo} 16-bit halves for the exact FP64 forms
+void bconv_constants(const Context &c, const std::vector<int> &src,
+                     const std::vector<int> &dst,
+                     const std::vector<int> &restore,
+                     std::vector<uint32_t> &c1, std::vector<int32_t> &table,
+                     std::vector<double> &ftab);
+
 // ---- launchers ----
 void launch_ntt_fwd(const Context &ctx, const NttLaunch &L, hipStream_t st);
 void launch_intt(const Context &ctx, const NttLaunch &L, hipStream_t st);

@@ -159,6 +159,10 @@ int plan_finish(Plan &p, bool mismatch) {
   return kOk;
 }
 
+int plan_upload(Plan &p, double **dst, const std::vector<double> &v) {
+  return upload_vec(p, dst, v);
+}
+
 int plan_alloc_workspace(Plan &p, size_t limbs, uint32_t **ws) {
   *ws = nullptr;
   if (limbs == 0) return kOk;

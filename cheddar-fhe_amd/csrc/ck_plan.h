@@ -60,6 +60,8 @@ class PlanSink : public Sink {
 };
 
 int plan_alloc_workspace(Plan &p, size_t limbs, uint32_t **ws);
+// a constant table the plan owns (freed by plan_destroy)
+int plan_upload(Plan &p, double **dst, const std::vector<double> &v);
 int plan_finish(Plan &p, bool mismatch);
 int plan_run(const Plan &p, hipStream_t st);
 void plan_destroy(Plan *p);

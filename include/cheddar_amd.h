@@ -306,6 +306,24 @@ int ck_plan_hrot_hoisted(ck_context *ctx, int n, const int *levels,
                          const uint32_t *const *key_b,
                          const uint32_t *const *key_a, uint32_t *const *out_b,
                          uint32_t *const *out_a, int moddown, ck_plan **out);
+/* out dst[k] = fast base conversion of src[k] on the FP64 matrix cores,
+ * k < n: the reference's ModSwitchMatrixMult (src/core/ModSwitch.cu:23-171)
+ * with the PopulateModSwitchConstants tables (:320-366) as a standalone
+ * step. From n_src (1..16) source primes s_i to n_dst (1..64) destination
+ * primes t_m (global prime indices):
+ *   dst[k] limb m = [sum_i v_i (prod_{j != i} s_j) R]_{t_m}  (R = 2^32, canonical)
+ * for src[k] = n_src int32 limbs v_i centred in [-(s_i - 1)/2, (s_i - 1)/2]
+ * (the INTT-and-constant output of a ModUp or ModDown), exactly the
+ * reference's ReduceMontgomery(sum_i v_i table_mi). One launch per run. */
+int ck_plan_bconv(ck_context *ctx, const int *src_primes, int n_src,
+                  const int *dst_primes, int n_dst, int n, const int32_t *const *src,
+                  uint32_t *const *dst, ck_plan **out);
+/* dst[k] = ck_ntt (inverse = 0) or ck_intt (inverse = 1) of src[k], k < n,
+ * all with one NP (NTTHandler::NTT / INTT over a batch): one launch per pass
+ * across the batch. */
+int ck_plan_ntt(ck_context *ctx, int n, int num_main, int num_ter, int num_aux,
+                int inverse, int montgomery_conversion, const uint32_t *const *src,
+                uint32_t *const *dst, ck_plan **out);
 int ck_plan_run(ck_plan *plan, void *stream);
 /* out[8]: items, steps, kernel launches per run, workspace bytes, BaseConv
  * form (1: matrix cores, 0: fused), and with autotuning the measured run

@@ -279,6 +279,57 @@ def test_gs_accum(ck, ctx24, oracle_rns24, nbs, ngs):
         ctx24.gs_accum([[dx[0][0], outs[0][1]]] + outs[1:], dx, dp, np3)
 
 
+@pytest.mark.parametrize("npoly", [1, 6])
+def test_bconv_plan_8_to_32(ck, ctx24, oracle_rns24, npoly):
+    """standalone BaseConv plan (ModSwitchMatrixMult, src/core/ModSwitch.cu:
+    23-171) from the 8 special primes to all 32 primes (the 8 -> 32 limb
+    contraction) over several polynomials in one launch: equal to the
+    oracle's table-driven conversion for centred sources, including the
+    extreme values +-(s - 1) / 2"""
+    P = oracle_rns24
+    src_p = list(range(24, 32))
+    dst_p = list(range(32))
+    _, table = P.bconv_tables(src_p, dst_p)
+    rng = np.random.default_rng(SEED_BASE + 700 + npoly)
+    srcs = []
+    for k in range(npoly):
+        v = np.empty((len(src_p), P.n), dtype=np.int32)
+        for i, pi in enumerate(src_p):
+            h = (P.prime(pi) - 1) // 2
+            v[i] = rng.integers(-h, h + 1, size=P.n, dtype=np.int64)
+            v[i, :2] = (h, -h)
+        srcs.append(v)
+    d_src = [torch.from_numpy(v).to("cuda") for v in srcs]
+    d_dst = [torch.full((len(dst_p), P.n), -1, dtype=torch.int32, device="cuda")
+             for _ in range(npoly)]
+    plan = ck.Plan.bconv(ctx24, src_p, dst_p, d_src, d_dst)
+    plan.run()
+    sync()
+    for k in range(npoly):
+        want = P.bconv(srcs[k], src_p, dst_p, table)
+        assert_same(ck.to_host(d_dst[k]), want, f"bconv poly {k}")
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+def test_ntt_plan_batch(ck, ctx24, oracle_rns24, inverse):
+    """batched NTT / INTT plan (one launch per pass over 5 polynomials of all
+    32 limbs) equal to the oracle's transform of each"""
+    P = oracle_rns24
+    pidx = P.np_primes(24, 0, 8)
+    xs = [uniform(P, pidx, SEED_BASE + 750 + k) for k in range(5)]
+    d_src = [ck.to_device(x) for x in xs]
+    d_dst = [torch.empty_like(d) for d in d_src]
+    plan = ck.Plan.ntt(ctx24, d_src, d_dst, (24, 0, 8), inverse=inverse, mont_conv=inverse)
+    info = plan.info()
+    assert info["items"] == 5 and info["launches"] == 2
+    plan.run()
+    sync()
+    for k, x in enumerate(xs):
+        want = (P.intt_np(x, 24, 0, 8, mont_conv=True) if inverse
+                else P.ntt_np(x, 24, 0, 8, mont_conv=False))
+        assert_same(ck.to_host(d_dst[k]), want, f"plan transform {k}")
+
+
 @pytest.mark.parametrize("num_aux", [0, 12])
 def test_boot40_modup_to_max(ck, ctx40, oracle_boot40, num_aux):
     P = oracle_boot40
