@@ -102,6 +102,20 @@ __device__ __forceinline__ double mulmod_f(double a, double w, double wq,
   return __builtin_fma(-k, q, hi) + lo;
 }
 
+// mulmod_f for a multiplier that changes from product to product (twiddles,
+// key words, data): the quotient comes from the rounded product,
+// k = rint(fl(a w) qr), qr = 1/q rounded. That is three roundings, as in
+// a * fl(w qr), so k is within 1/2 + 3 |a w / q| 2^-53 of a w / q and
+// |r| < q/2 + 3 |a w| 2^-53 (< q/2 + 2^20 for |a| < 2^40); the
+// exactness argument of mulmod_f is unchanged, and no w/q is formed.
+__device__ __forceinline__ double mulmod_fq(double a, double w, double q,
+                                            double qr) {
+  double hi = a * w;
+  double lo = __builtin_fma(a, w, -hi);
+  double k = __builtin_rint(hi * qr);
+  return __builtin_fma(-k, q, hi) + lo;
+}
+
 // |x| < 2^50 -> x mod q in (-0.75 q, 0.75 q) (one rounding step; within
 // (-q/2 - 1, q/2 + 1) for |x| < 2^40)
 __device__ __forceinline__ double reduce_f(double x, const ModF &m) {
@@ -109,9 +123,14 @@ __device__ __forceinline__ double reduce_f(double x, const ModF &m) {
   return __builtin_fma(-k, m.q, x);
 }
 
-// |r| < q (a mulmod_f or reduce_f result) -> canonical [0, q) as uint32
+// |r| < q (a mulmod_f or reduce_f result) -> canonical [0, q) as uint32.
+// r converts to int32 exactly; as unsigned u, u + q wraps to r + q in
+// (0, q) when r < 0 and exceeds u = r otherwise (q < 2^31), so the unsigned
+// minimum is the canonical residue.
 __device__ __forceinline__ uint32_t canon_small(double r, double q) {
-  return (uint32_t)(r < 0.0 ? r + q : r);
+  const uint32_t u = (uint32_t)(int32_t)r;
+  const uint32_t v = u + (uint32_t)q;
+  return u < v ? u : v;
 }
 
 // |x| < 2^50 -> canonical [0, q) as uint32

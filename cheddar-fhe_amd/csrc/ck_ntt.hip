@@ -56,12 +56,11 @@ __device__ __forceinline__ void ct16(double x[16], const uint32_t tw[15],
 #pragma unroll
     for (int blk = 0; blk < (1 << s); blk++) {
       const double w = (double)tw[(1 << s) - 1 + blk];
-      const double wq = w * qr;
 #pragma unroll
       for (int j = 0; j < half; j++) {
         double &a = x[blk * 2 * half + j];
         double &b = x[blk * 2 * half + j + half];
-        const double r = mulmod_f(b, w, wq, q);
+        const double r = mulmod_fq(b, w, q, qr);
         b = a - r;
         a = a + r;
       }
@@ -81,12 +80,11 @@ __device__ __forceinline__ void ct16_lds(double x[16], const uint32_t *tws,
 #pragma unroll
     for (int blk = 0; blk < (1 << s); blk++) {
       const double w = (double)tws[((1 << s) - 1 + blk) * 256 + threadIdx.x];
-      const double wq = w * qr;
 #pragma unroll
       for (int j = 0; j < half; j++) {
         double &a = x[blk * 2 * half + j];
         double &b = x[blk * 2 * half + j + half];
-        const double r = mulmod_f(b, w, wq, q);
+        const double r = mulmod_fq(b, w, q, qr);
         b = a - r;
         a = a + r;
       }
@@ -104,14 +102,13 @@ __device__ __forceinline__ void gs16(double x[16], const uint32_t tw[15],
 #pragma unroll
     for (int blk = 0; blk < (8 >> u); blk++) {
       const double w = (double)tw[16 - (16 >> u) + blk];
-      const double wq = w * qr;
 #pragma unroll
       for (int j = 0; j < half; j++) {
         double &a = x[blk * 2 * half + j];
         double &b = x[blk * 2 * half + j + half];
         const double d = a - b;
         a = a + b;
-        b = mulmod_f(d, w, wq, q);
+        b = mulmod_fq(d, w, q, qr);
       }
     }
   }
@@ -583,15 +580,17 @@ __global__ __launch_bounds__(kColThreads, 5) void intt_cols(
   uint32_t *__restrict__ o = J.dst + c0 + cc;
   // x * c1 lands in (-0.75 q, 0.75 q): canonical [0, q) (above: never,
   // below 0: + q) or centred [-(q-1)/2, (q-1)/2] (one fix-up either side)
+  // (integer fix-ups on the exact int32 conversion)
+  const int32_t qi = (int32_t)P.q[J.prime];
   const bool centred = (J.flags & kInttCentred) != 0;
-  const double above = centred ? (q - 1.0) * 0.5 : q;
-  const double below = centred ? -(q - 1.0) * 0.5 : 0.0;
+  const int32_t above = centred ? (qi - 1) >> 1 : qi - 1;
+  const int32_t below = centred ? -((qi - 1) >> 1) : 0;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
-    double r = mulmod_f(x[k], cv, cq, q);
-    r = r > above ? r - q : r;
-    r = r < below ? r + q : r;
-    o[(rb + 16 * k) * 256] = (uint32_t)(int32_t)r;
+    int32_t v = (int32_t)mulmod_f(x[k], cv, cq, q);
+    v = v > above ? v - qi : v;
+    v = v < below ? v + qi : v;
+    o[(rb + 16 * k) * 256] = (uint32_t)v;
   }
 }
 
@@ -693,7 +692,7 @@ __device__ __forceinline__ void key_rows_body(const EwNp &np, const PrimeTab &P,
 #pragma unroll
           for (int e = 0; e < 4; e++) {
             const double vd = (double)V[e];
-            const double t = mulmod_f((double)U[e], vd, vd * qr, q);
+            const double t = mulmod_fq((double)U[e], vd, q, qr);
             x[4 * i + e] = mulmod_f(t, rinv, rinvq, q);
           }
         }
@@ -730,8 +729,8 @@ __device__ __forceinline__ void key_rows_body(const EwNp &np, const PrimeTab &P,
 #pragma unroll
       for (int e = 0; e < 4; e++) {
         const double b = (double)KB[e], a = (double)KA[e];
-        sb[4 * i + e] += mulmod_f(x[4 * i + e], b, b * qr, q);
-        sa[4 * i + e] += mulmod_f(x[4 * i + e], a, a * qr, q);
+        sb[4 * i + e] += mulmod_fq(x[4 * i + e], b, q, qr);
+        sa[4 * i + e] += mulmod_fq(x[4 * i + e], a, q, qr);
       }
     }
   }
@@ -752,8 +751,8 @@ __device__ __forceinline__ void key_rows_body(const EwNp &np, const PrimeTab &P,
       for (int e = 0; e < 4; e++) {
         const double db1 = (double)B1[e], da1 = (double)A1[e];
         const double db2 = (double)B2[e], da2 = (double)A2[e];
-        const double d0 = mulmod_f(db1, db2, db2 * qr, q);
-        const double d1 = mulmod_f(db1, da2, da2 * qr, q) + mulmod_f(da1, db2, db2 * qr, q);
+        const double d0 = mulmod_fq(db1, db2, q, qr);
+        const double d1 = mulmod_fq(db1, da2, q, qr) + mulmod_fq(da1, db2, q, qr);
         sb[4 * i + e] += mulmod_f(d0, pq, pqq, q);
         sa[4 * i + e] += mulmod_f(d1, pq, pqq, q);
       }
