@@ -22,7 +22,8 @@ namespace ck {
 // Montgomery (R = 2^32), signed lazy, as Basic.cuh:90-99 defines it
 // ---------------------------------------------------------------------------
 
-// |a| <= q * 2^31  ->  a * 2^-32 mod q in (-q, q)
+// |a| < q * 2^31  ->  a * 2^-32 mod q in (-q, q): the result is
+// (a - t q) / 2^32 with |t| <= 2^31 (strict bound needed: a = q 2^31 gives q)
 __device__ __forceinline__ int32_t mont_reduce_lazy(int64_t a, uint32_t q,
                                                     int32_t qinv) {
   int32_t lo = (int32_t)(uint32_t)(uint64_t)a;
@@ -39,7 +40,7 @@ __device__ __forceinline__ uint32_t mont_reduce(int64_t a, uint32_t q,
   return (uint32_t)r;
 }
 
-// a, b in (-2^31, 2^31) with |a*b| <= q*2^31: a * b * 2^-32 mod q in (-q,q)
+// a, b in (-2^31, 2^31) with |a*b| < q*2^31: a * b * 2^-32 mod q in (-q,q)
 __device__ __forceinline__ int32_t mont_mul_lazy(int32_t a, int32_t b,
                                                  uint32_t q, int32_t qinv) {
   return mont_reduce_lazy((int64_t)a * (int64_t)b, q, qinv);
@@ -90,10 +91,12 @@ struct ModF {
 };
 
 // a * w mod q for integer-valued doubles; |a| < 2^50, 0 <= w < q < 2^31,
-// wq = w/q (correctly rounded). Result r == a*w (mod q) exactly with
-// |r| < q/2 + |a w| 2^-52 < 0.75 q: k is within 1/2 + |a w / q| 2^-52 of
-// a w / q, hi - k q is an integer below 2^30 in magnitude (exact in the
-// fma) and lo is the exact rounding error of hi.
+// wq = fl(w qr) with qr = 1/q rounded. Result r == a*w (mod q) exactly with
+// |r| < q/2 + 3 |a w| 2^-53 (< q/2 + 2^20 for |a| < 2^40): k (three
+// roundings) is within 1/2 + 3 |a w / q| 2^-53 of a w / q, hi - k q is an
+// integer small enough to be exact in the fma, and lo is the exact
+// rounding error of hi. tests/cpp/test_device_arith.cpp checks this and
+// every other function here against 128-bit integer arithmetic.
 __device__ __forceinline__ double mulmod_f(double a, double w, double wq,
                                            double q) {
   double hi = a * w;
