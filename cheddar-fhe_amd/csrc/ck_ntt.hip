@@ -412,6 +412,16 @@ __global__ __launch_bounds__(kRowThreads, 5) void ntt_fwd_rows(
   // the pass-1 intermediate (in dst) is lazy signed int32
   double x[16];
   rows_load_p1(J.dst + tile, rl, t16, x);
+  // the ModDown epilogue's second operand, issued with the data loads so its
+  // latency overlaps the row stages. Unconditional, since a conditional form
+  // costs a spill here: without an epilogue every lane reads the same first
+  // 64 bytes of the limb (one cached line per wave), unused.
+  const bool has2 = (J.flags & kNttModDownEpilogue) && J.src2;
+  const uint4 *s2p = reinterpret_cast<const uint4 *>(
+      has2 ? J.src2 + (size_t)r * 256 + t16 * 16 : J.dst);
+  uint4 s2[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) s2[i] = s2p[i];
   ct16(x, t1, q, qr);
 #pragma unroll
   for (int k = 0; k < 16; k++) ldsd[rlds(rl, t16 + 16 * k)] = x[k];
@@ -426,11 +436,10 @@ __global__ __launch_bounds__(kRowThreads, 5) void ntt_fwd_rows(
     // NTTPhase2ForModDown (NTT.cu:456-601): out = (src2 * pad - y) * ip
     const double ip = (double)J.c3, ipq = ip * qr;
     double a2[16];
-    if (J.src2) {
-      const uint32_t *__restrict__ sp = J.src2 + r * 256 + t16 * 16;
+    if (has2) {
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const uint4 u = reinterpret_cast<const uint4 *>(sp)[i];
+        const uint4 u = s2[i];
         a2[4 * i + 0] = (double)u.x;
         a2[4 * i + 1] = (double)u.y;
         a2[4 * i + 2] = (double)u.z;
