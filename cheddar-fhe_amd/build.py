@@ -26,6 +26,14 @@ COMMON = [
 ]
 
 
+# per-file flags: the int8 matrix-core kernels keep their MFMA results in
+# ordinary VGPRs (the default allocation parks them in accumulation
+# registers and copies every result back with v_accvgpr_read)
+EXTRA = {
+    "ck_i8.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+}
+
+
 def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC)
                   if f.endswith(".hip") or f.endswith(".cpp"))
@@ -33,7 +41,7 @@ def sources():
 
 def compile_one(src):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC] + COMMON + ["-c", src, "-o", obj]
+    cmd = [HIPCC] + COMMON + EXTRA.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
     if src.endswith(".cpp"):
         cmd[1:1] = ["-x", "hip"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -61,24 +69,32 @@ def build(jobs=8, verbose=False):
 
 
 BOOT_BENCH = os.path.join(HERE, "boot_bench")
+PLAN_BENCH = os.path.join(HERE, "plan_bench")
 
 
-def build_boot_bench(verbose=False):
-    """The bootstrapping benchmark (host/boot_bench.cpp) over the C++ facade,
+def build_host_program(name, out, verbose=False):
+    """A host program of host/ (C++ over the facade headers and the C ABI),
     linked against the library just built."""
     repo = os.path.dirname(HERE)
     cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
            "-I/opt/rocm/include", "-I" + os.path.join(repo, "include"),
            "-I" + os.path.join(HERE, "host", "include"),
-           os.path.join(HERE, "host", "boot_bench.cpp"), "-o", BOOT_BENCH + ".tmp",
+           os.path.join(HERE, "host", name), "-o", out + ".tmp",
            "-L" + HERE, "-lcheddar_amd", "-L/opt/rocm/lib", "-lamdhip64",
            "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"boot_bench build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(BOOT_BENCH + ".tmp", BOOT_BENCH)
+        raise RuntimeError(f"{name} build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(out + ".tmp", out)
     if verbose:
-        print(f"built {BOOT_BENCH}")
+        print(f"built {out}")
+
+
+def build_boot_bench(verbose=False):
+    """The bootstrapping benchmark (host/boot_bench.cpp) and the batched-plan
+    benchmark (host/plan_bench.cpp, no torch in the process)."""
+    build_host_program("boot_bench.cpp", BOOT_BENCH, verbose)
+    build_host_program("plan_bench.cpp", PLAN_BENCH, verbose)
 
 
 if __name__ == "__main__":

@@ -19,6 +19,13 @@ CK_MODDOWN, CK_RESCALE, CK_MODDOWN_RESCALE = 0, 1, 2
 CK_OPT_BCONV_MFMA = 1
 CK_OPT_PLAN_AUTOTUNE = 2
 CK_OPT_FUSED_KEY_ROWS = 3
+CK_OPT_PLAN_NTT_FORM = 4
+CK_OPT_PLAN_BCONV_FORM = 5
+CK_OPT_NTT8_BLOCKS = 6
+CK_OPT_NTT8_CHUNK = 7
+# forms of transform / BaseConv plans (CK_OPT_PLAN_NTT_FORM, CK_OPT_PLAN_BCONV_FORM)
+NTT_FORMS = {"valu": 0, "int8": 1, "auto": -1}
+BCONV_FORMS = {"fp64": 0, "int8": 1, "valu": 2, "auto": -1}
 
 _vp = ctypes.c_void_p
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -102,6 +109,7 @@ _SIGS = {
     "ck_plan_run": (ctypes.c_int, [_vp, _vp]),
     "ck_plan_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
     "ck_plan_destroy": (ctypes.c_int, [_vp]),
+    "ck_plan_tuning": (ctypes.c_int, [_vp, ctypes.c_int, _ip, ctypes.POINTER(ctypes.c_double)]),
     "ck_context_set_option": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "ck_gs_accum": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
@@ -406,6 +414,20 @@ class Context:
         v = -1 if on is None else int(bool(on))
         _check(lib.ck_context_set_option(self._h, CK_OPT_BCONV_MFMA, v))
 
+    def set_plan_forms(self, ntt=None, bconv=None, ntt8_blocks=None, ntt8_chunk=None):
+        """Forms of ck_plan_ntt forward transforms ("valu", "int8", "auto")
+        and of ck_plan_bconv ("fp64", "int8", "valu", "auto"), and the int8
+        transform's blocks per workgroup / chunk; outputs are identical."""
+        if ntt is not None:
+            _check(lib.ck_context_set_option(self._h, CK_OPT_PLAN_NTT_FORM, NTT_FORMS[ntt]))
+        if bconv is not None:
+            _check(lib.ck_context_set_option(self._h, CK_OPT_PLAN_BCONV_FORM,
+                                             BCONV_FORMS[bconv]))
+        if ntt8_blocks is not None:
+            _check(lib.ck_context_set_option(self._h, CK_OPT_NTT8_BLOCKS, int(ntt8_blocks)))
+        if ntt8_chunk is not None:
+            _check(lib.ck_context_set_option(self._h, CK_OPT_NTT8_CHUNK, int(ntt8_chunk)))
+
     def set_fused_key_rows(self, on=True):
         """Key switches run the ModUp's forward row stages and the key inner
         product as one pass (True, default) or as a row pass plus a separate
@@ -495,7 +517,20 @@ class Plan:
         _check(lib.ck_plan_info(self._h, out))
         return {"items": out[0], "steps": out[1], "launches": out[2],
                 "workspace_bytes": out[3], "bconv_mfma": bool(out[4]),
-                "tuned_us": (out[5] / 1e3, out[6] / 1e3) if out[5] != -1000 else None}
+                "tuned_us": (out[5] / 1e3, out[6] / 1e3) if out[5] != -1000 else None,
+                "form": out[7]}
+
+    def tuning(self):
+        """[(form, int8 blocks, int8 chunk, us)] of every autotuned candidate"""
+        k = lib.ck_plan_tuning(self._h, 0, None, None)
+        if k < 0:
+            raise CheddarError(lib.ck_last_error().decode())
+        if k == 0:
+            return []
+        forms = (ctypes.c_int * (3 * k))()
+        us = (ctypes.c_double * k)()
+        lib.ck_plan_tuning(self._h, k, forms, us)
+        return [(forms[3 * i], forms[3 * i + 1], forms[3 * i + 2], us[i]) for i in range(k)]
 
     def __del__(self):
         h = getattr(self, "_h", None)

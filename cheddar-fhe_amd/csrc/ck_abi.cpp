@@ -454,6 +454,9 @@ int ck_permute_accum(ck_context *ctx, int num_main, int num_ter, int num_aux, in
     for (int j = 0; j < npoly; j++)
       CK_REQUIRE(src[s] == nullptr || src[s] != dst[j],
                  "PermuteAccum does not support a source aliasing the output");
+  // no sources: accumulating leaves dst as it is; otherwise one pass of
+  // masked terms writes zeros
+  if (nsrc == 0 && accumulate) return CK_OK;
   const EwNp np = make_np(c, num_main, num_ter, num_aux);
   int done = 0;
   do {
@@ -734,6 +737,47 @@ int ck_plan_hmult(ck_context *ctx, int n, const int *levels, int rescale,
   return CK_OK;
 }
 
+// Builds every candidate, times each (one warm-up + 3 runs: the runs write
+// the plan's outputs), keeps the fastest and records all times. A single
+// candidate is built and returned untimed.
+extern "C++" template <class Build>
+static int tune_candidates(const std::vector<Plan::Candidate> &cands, Build &build,
+                           Plan **out) {
+  *out = nullptr;
+  if (cands.size() == 1) return build(cands[0], out);
+  Plan *best = nullptr;
+  double best_us = -1.0;
+  std::vector<Plan::Candidate> rec;
+  for (Plan::Candidate cd : cands) {
+    Plan *p = nullptr;
+    const int rc = build(cd, &p);
+    if (rc) {
+      plan_destroy(best);
+      return rc;
+    }
+    cd.us = plan_time_us(*p, 3);
+    rec.push_back(cd);
+    if (cd.us >= 0 && (best_us < 0 || cd.us < best_us)) {
+      plan_destroy(best);
+      best = p;
+      best_us = cd.us;
+    } else {
+      plan_destroy(p);
+    }
+  }
+  if (!best) return fail(CK_ERR_HIP, "autotuning could not time any candidate plan");
+  best->tuning = rec;
+  *out = best;
+  return CK_OK;
+}
+
+extern "C++" bool aliases(const void *const *a, int na, const void *const *b, int nb) {
+  for (int i = 0; i < na; i++)
+    for (int j = 0; j < nb; j++)
+      if (a[i] == b[j]) return true;
+  return false;
+}
+
 int ck_plan_bconv(ck_context *ctx, const int *src_primes, int n_src,
                   const int *dst_primes, int n_dst, int n, const int32_t *const *src,
                   uint32_t *const *dst, ck_plan **out) {
@@ -753,39 +797,94 @@ int ck_plan_bconv(ck_context *ctx, const int *src_primes, int n_src,
     CK_REQUIRE(dst_primes[m] >= 0 && dst_primes[m] < c.num_primes,
                "invalid destination prime");
   for (int k = 0; k < n; k++) CK_REQUIRE(src[k] && dst[k], "null pointer");
+  CK_REQUIRE(c.plan_bconv_form >= 0 ||
+                 !aliases((const void *const *)src, n, (const void *const *)dst, n),
+             "autotuned plans need outputs that do not alias inputs");
   std::vector<int> sp(src_primes, src_primes + n_src), dp(dst_primes, dst_primes + n_dst);
   std::vector<uint32_t> c1;
   std::vector<int32_t> table;
   std::vector<double> ftab;
   bconv_constants(c, sp, dp, {}, c1, table, ftab);
-  Plan *p = new Plan;
-  p->ctx = &c;
-  p->items = n;
-  p->bconv_mfma = true;
-  double *d_tab = nullptr;
-  int rc = plan_upload(*p, &d_tab, ftab);
-  PlanSink sink(*p);
-  for (int k = 0; k < n && !rc; k++) {
-    sink.begin_item();
-    BconvInst I;
-    std::memset(&I, 0, sizeof(I));
-    I.src = src[k];
-    I.dst = dst[k];
-    I.tab = d_tab;
-    I.S = n_src;
-    I.D = n_dst;
-    I.canonical = 1;
-    for (int m = 0; m < n_dst; m++) {
-      I.dst_limb[m] = (uint16_t)m;
-      I.dst_prime[m] = (uint8_t)dp[m];
+  auto build = [&](const Plan::Candidate &cd, Plan **res) {
+    Plan *p = new Plan;
+    p->ctx = &c;
+    p->items = n;
+    p->form = cd.form;
+    int rc = kOk;
+    if (cd.form == 0) {  // FP64 matrix cores
+      p->bconv_mfma = true;
+      double *d_tab = nullptr;
+      rc = plan_upload(*p, &d_tab, ftab);
+      PlanSink sink(*p);
+      for (int k = 0; k < n && !rc; k++) {
+        sink.begin_item();
+        BconvInst I;
+        std::memset(&I, 0, sizeof(I));
+        I.src = src[k];
+        I.dst = dst[k];
+        I.tab = d_tab;
+        I.S = n_src;
+        I.D = n_dst;
+        I.canonical = 1;
+        for (int m = 0; m < n_dst; m++) {
+          I.dst_limb[m] = (uint16_t)m;
+          I.dst_prime[m] = (uint8_t)dp[m];
+        }
+        rc = sink.bconv({I});
+      }
+      if (!rc) rc = plan_finish(*p, sink.mismatch());
+    } else {  // int8 matrix cores (1) or vector unit (2)
+      int64_t *d_planes = nullptr;
+      double *d_ftab = nullptr;
+      if (cd.form == 1) {
+        std::vector<uint32_t> tab((size_t)n_dst * n_src), dq(n_dst);
+        for (int m = 0; m < n_dst; m++) {
+          dq[m] = c.primes[dp[m]];
+          for (int i = 0; i < n_src; i++)
+            tab[(size_t)m * n_src + i] =
+                (uint32_t)ftab[2 * (m * n_src + i)] * 65536u +
+                (uint32_t)ftab[2 * (m * n_src + i) + 1];
+        }
+        rc = plan_upload_i64(*p, &d_planes, i8::bconv_i8_planes(tab, n_src, n_dst, dq));
+      } else {
+        rc = plan_upload(*p, &d_ftab, ftab);
+      }
+      p->stages.emplace_back();
+      PlanStage &s = p->stages.back();
+      s.kind = cd.form == 1 ? PlanStage::kBconvI8 : PlanStage::kBconvValu;
+      for (int k = 0; k < n; k++) {
+        BconvI8Inst I;
+        std::memset(&I, 0, sizeof(I));
+        I.src = src[k];
+        I.dst = dst[k];
+        I.planes = d_planes;
+        I.ftab = d_ftab;
+        I.S = n_src;
+        I.D = n_dst;
+        I.canonical = 1;
+        for (int m = 0; m < n_dst; m++) {
+          I.dst_limb[m] = (uint16_t)m;
+          I.dst_prime[m] = (uint8_t)dp[m];
+        }
+        s.bi.push_back(I);
+      }
+      if (!rc) rc = plan_finish(*p, false);
     }
-    rc = sink.bconv({I});
-  }
-  if (!rc) rc = plan_finish(*p, sink.mismatch());
-  if (rc) {
-    plan_destroy(p);
+    if (rc) {
+      plan_destroy(p);
+      p = nullptr;
+    }
+    *res = p;
     return rc;
-  }
+  };
+  std::vector<Plan::Candidate> cands;
+  if (c.plan_bconv_form >= 0)
+    cands.push_back({c.plan_bconv_form, 0, 0, -1.0});
+  else
+    for (int f = 0; f < 3; f++) cands.push_back({f, 0, 0, -1.0});
+  Plan *p = nullptr;
+  const int rc = tune_candidates(cands, build, &p);
+  if (rc) return rc;
   *out = new ck_plan{p};
   return CK_OK;
 }
@@ -796,27 +895,69 @@ int ck_plan_ntt(ck_context *ctx, int n, int num_main, int num_ter, int num_aux,
   CK_REQUIRE(out, "out is null");
   *out = nullptr;
   CK_REQUIRE(ctx && ctx->c && src && dst, "null argument");
-  const Context &c = *ctx->c;
+  Context &c = *ctx->c;
   CK_REQUIRE(n >= 1, "a plan needs at least one polynomial");
   CK_REQUIRE(valid_np(c, num_main, num_ter, num_aux), "invalid NP");
   for (int k = 0; k < n; k++) CK_REQUIRE(src[k] && dst[k], "null pointer");
-  Plan *p = new Plan;
-  p->ctx = &c;
-  p->items = n;
-  PlanSink sink(*p);
-  int rc = kOk;
-  for (int k = 0; k < n && !rc; k++) {
-    sink.begin_item();
-    rc = inverse ? op_intt(c, dst[k], src[k], num_main, num_ter, num_aux, 0,
-                           montgomery_conversion != 0, sink)
-                 : op_ntt(c, dst[k], src[k], num_main, num_ter, num_aux, 0,
-                          montgomery_conversion != 0, sink);
+  // the int8 matrix-core form covers the forward transform
+  const int form = inverse ? 0 : c.plan_ntt_form;
+  CK_REQUIRE(form >= 0 || !aliases((const void *const *)src, n, (const void *const *)dst, n),
+             "autotuned plans need outputs that do not alias inputs");
+  const int limbs = num_main + num_ter + num_aux;
+  if (form != 0) {
+    I8NttTabs T;
+    const int rc = ntt8_tables(c, &T);
+    if (rc) return rc;
   }
-  if (!rc) rc = plan_finish(*p, sink.mismatch());
-  if (rc) {
-    plan_destroy(p);
+  auto build = [&](const Plan::Candidate &cd, Plan **res) {
+    Plan *p = new Plan;
+    p->ctx = &c;
+    p->items = n;
+    p->form = cd.form;
+    int rc = kOk;
+    if (cd.form == 0) {
+      PlanSink sink(*p);
+      for (int k = 0; k < n && !rc; k++) {
+        sink.begin_item();
+        rc = inverse ? op_intt(c, dst[k], src[k], num_main, num_ter, num_aux, 0,
+                               montgomery_conversion != 0, sink)
+                     : op_ntt(c, dst[k], src[k], num_main, num_ter, num_aux, 0,
+                              montgomery_conversion != 0, sink);
+      }
+      if (!rc) rc = plan_finish(*p, sink.mismatch());
+    } else {
+      rc = ntt8_tables(c, &p->i8);
+      std::vector<Ntt8Job> jobs;
+      for (int k = 0; k < n; k++)
+        for (int y = 0; y < limbs; y++)
+          jobs.push_back(Ntt8Job{src[k] + (size_t)y * kLimb, dst[k] + (size_t)y * kLimb,
+                                 global_prime(c, num_main, num_ter, y),
+                                 montgomery_conversion != 0 ? 1 : 0});
+      if (!rc) {
+        plan_add_ntt8(*p, std::move(jobs), cd.blocks, cd.chunk);
+        rc = plan_finish(*p, false);
+      }
+    }
+    if (rc) {
+      plan_destroy(p);
+      p = nullptr;
+    }
+    *res = p;
     return rc;
+  };
+  std::vector<Plan::Candidate> cands;
+  if (form >= 0) {
+    cands.push_back({form, c.ntt8_blocks, c.ntt8_chunk, -1.0});
+  } else {
+    cands.push_back({0, 0, 0, -1.0});
+    const int total = n * limbs;
+    for (int blocks : {16, 8, 4})
+      for (int chunk : {0, 512, 256, 128})
+        if (chunk == 0 || chunk < total) cands.push_back({1, blocks, chunk, -1.0});
   }
+  Plan *p = nullptr;
+  const int rc = tune_candidates(cands, build, &p);
+  if (rc) return rc;
   *out = new ck_plan{p};
   return CK_OK;
 }
@@ -892,8 +1033,24 @@ int ck_plan_info(const ck_plan *plan, long long *out) {
   out[4] = plan->p->bconv_mfma ? 1 : 0;
   out[5] = (long long)(plan->p->tuned_us[0] * 1000.0);  // ns, -1000: not tuned
   out[6] = (long long)(plan->p->tuned_us[1] * 1000.0);
-  out[7] = 0;
+  out[7] = plan->p->form;
   return CK_OK;
+}
+
+int ck_plan_tuning(const ck_plan *plan, int max, int *forms, double *us) {
+  if (!plan || !plan->p || max < 0 || (max && (!forms || !us))) {
+    set_error("invalid plan tuning arguments");
+    return -1;
+  }
+  const std::vector<Plan::Candidate> &t = plan->p->tuning;
+  const int k = (int)t.size() < max ? (int)t.size() : max;
+  for (int i = 0; i < k; i++) {
+    forms[3 * i] = t[i].form;
+    forms[3 * i + 1] = t[i].blocks;
+    forms[3 * i + 2] = t[i].chunk;
+    us[i] = t[i].us;
+  }
+  return (int)t.size();
 }
 
 int ck_plan_destroy(ck_plan *plan) {
@@ -917,6 +1074,23 @@ int ck_context_set_option(ck_context *ctx, int option, int value) {
     case CK_OPT_FUSED_KEY_ROWS:
       CK_REQUIRE(value == 0 || value == 1, "option value must be 0 or 1");
       ctx->c->fuse_key_rows = value != 0;
+      return CK_OK;
+    case CK_OPT_PLAN_NTT_FORM:
+      CK_REQUIRE(value >= -1 && value <= 1, "option value must be -1, 0 or 1");
+      ctx->c->plan_ntt_form = value;
+      return CK_OK;
+    case CK_OPT_PLAN_BCONV_FORM:
+      CK_REQUIRE(value >= -1 && value <= 2, "option value must be -1, 0, 1 or 2");
+      ctx->c->plan_bconv_form = value;
+      return CK_OK;
+    case CK_OPT_NTT8_BLOCKS:
+      CK_REQUIRE(value == 1 || value == 2 || value == 4 || value == 8 || value == 16,
+                 "option value must be 1, 2, 4, 8 or 16");
+      ctx->c->ntt8_blocks = value;
+      return CK_OK;
+    case CK_OPT_NTT8_CHUNK:
+      CK_REQUIRE(value >= 0, "option value must be >= 0");
+      ctx->c->ntt8_chunk = value;
       return CK_OK;
     default:
       return fail(CK_ERR_INVALID_ARG, "unknown context option");

@@ -385,6 +385,7 @@ int context_create(int log_n, const uint32_t *main_primes, int n_main,
   std::vector<int32_t> qinv(np);
   std::vector<double> qd(np), qr(np);
   std::vector<uint32_t> tw_f((size_t)np * n), tw_i((size_t)np * n), imag(np);
+  c->psi.assign(np, 0);
   for (int i = 0; i < np; i++) {
     const uint32_t q = all[i];
     qinv[i] = inv_mod_2_32(q);
@@ -393,6 +394,7 @@ int context_create(int log_n, const uint32_t *main_primes, int n_main,
     const uint32_t g = smallest_generator(q);
     const uint32_t psi = h_powmod(g, (q - 1) / (2u * n), q);
     const uint32_t psi_inv = h_invmod(psi, q);
+    c->psi[i] = psi;
     uint64_t pf = 1, pi = 1;
     for (int j = 0; j < n; j++) {
       const uint32_t r = brv((uint32_t)j, log_n);

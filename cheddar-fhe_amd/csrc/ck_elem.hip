@@ -549,7 +549,10 @@ __global__ __launch_bounds__(kThreads) void permute_kernel(
 
 // PermAccArgs: the NS (>= nsrc) source values of each polynomial are
 // loaded (a null source reads dst[0] and is masked to zero) before the
-// first addition; one body per source bound keeps dummy loads few
+// first addition; one body per source bound keeps dummy loads few. A
+// masked term past nsrc gathers with the identity index (g = 1 gives
+// idx = k), so its dummy load stays inside the limb also when nsrc = 0
+// (an unset g = 0 would give e g - 1 = -1, index 2^17 - 1).
 template <int NS>
 __global__ __launch_bounds__(kThreads) void permute_accum_kernel(const EwNp np,
                                                                  const PrimeTab P,
@@ -563,7 +566,7 @@ __global__ __launch_bounds__(kThreads) void permute_accum_kernel(const EwNp np,
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     const int ss = s < A.nsrc ? s : 0;
-    const uint32_t g = A.g[ss];
+    const uint32_t g = s < A.nsrc ? A.g[ss] : 1u;
     const uint32_t idx = g == 1u ? (uint32_t)k : __brev((e * g - 1u) & 0x1FFFFu) >> 15;
 #pragma unroll
     for (int j = 0; j < 2; j++) {

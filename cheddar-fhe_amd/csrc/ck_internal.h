@@ -141,6 +141,7 @@ struct Context {
   std::vector<int32_t> qinv;
   std::vector<uint32_t> p_prod_mont;  // P mod q_i * R (q primes)
   std::vector<uint32_t> galois;       // 5^i mod 2N, i in [0, N/2]
+  std::vector<uint32_t> psi;          // the 2N-th root of each prime's NTT
 
   // device tables
   uint32_t *d_q = nullptr;
@@ -151,6 +152,9 @@ struct Context {
   uint32_t *d_p_prod = nullptr;  // P mod q * R, indexed by global q prime
   uint32_t *d_imag = nullptr;    // psi^(N/2) * R (X^(N/2) in NTT form), per prime
   PrimeTab ptab{};
+  // int8 matrix-core transform tables (built on first use, ck_i8.hip)
+  const int32_t *i8_planes = nullptr, *i8_corr = nullptr;
+  const uint32_t *i8_tw = nullptr;
 
   std::vector<LevelPlan> levels;
 
@@ -164,6 +168,12 @@ struct Context {
   // product (one pass, no NTT-form ModUp output) instead of a row pass and
   // a separate key_accum
   bool fuse_key_rows = true;
+  // forms of ck_plan_ntt (forward) and ck_plan_bconv: see CK_OPT_PLAN_NTT_FORM
+  // and CK_OPT_PLAN_BCONV_FORM in cheddar_amd.h (-1: autotune)
+  int plan_ntt_form = 0;
+  int plan_bconv_form = 0;
+  int ntt8_blocks = 16;  // int8 transform: 16-column blocks per workgroup
+  int ntt8_chunk = 0;    // int8 transform: limbs per column/row chunk (0: all)
 
   // workspace for multi-kernel operations (one operation at a time per
   // context; concurrent calls on different streams race)

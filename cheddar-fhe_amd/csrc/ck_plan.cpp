@@ -145,7 +145,8 @@ int plan_finish(Plan &p, bool mismatch) {
   }
   for (PlanStage &s : p.stages) {
     order_by_prime(s);
-    if (s.ka.size() > 65535 || s.bs.size() > 65535 || s.bc.size() > 65535) {
+    if (s.ka.size() > 65535 || s.bs.size() > 65535 || s.bc.size() > 65535 ||
+        s.bi.size() > 65535) {
       set_error("too many items in one plan step");
       return kErrUnsupported;
     }
@@ -154,6 +155,8 @@ int plan_finish(Plan &p, bool mismatch) {
     if (!rc) rc = upload_vec(p, &s.d_bs, s.bs);
     if (!rc) rc = upload_vec(p, &s.d_ew, s.ew);
     if (!rc) rc = upload_vec(p, &s.d_bc, s.bc);
+    if (!rc) rc = upload_vec(p, &s.d_n8, s.n8);
+    if (!rc) rc = upload_vec(p, &s.d_bi, s.bi);
     if (rc) return rc;
   }
   return kOk;
@@ -161,6 +164,27 @@ int plan_finish(Plan &p, bool mismatch) {
 
 int plan_upload(Plan &p, double **dst, const std::vector<double> &v) {
   return upload_vec(p, dst, v);
+}
+
+int plan_upload_i64(Plan &p, int64_t **dst, const std::vector<int64_t> &v) {
+  return upload_vec(p, dst, v);
+}
+
+void plan_add_ntt8(Plan &p, std::vector<Ntt8Job> jobs, int blocks, int chunk) {
+  std::stable_sort(jobs.begin(), jobs.end(),
+                   [](const Ntt8Job &a, const Ntt8Job &b) { return a.prime < b.prime; });
+  const size_t n = jobs.size();
+  const size_t step = chunk > 0 ? (size_t)chunk : n;
+  for (size_t b = 0; b < n; b += step) {
+    const size_t e = std::min(n, b + step);
+    for (int pass = 0; pass < 2; pass++) {
+      p.stages.emplace_back();
+      PlanStage &s = p.stages.back();
+      s.kind = pass ? PlanStage::kNtt8Rows : PlanStage::kNtt8Cols;
+      s.blocks = blocks;
+      s.n8.assign(jobs.begin() + b, jobs.begin() + e);
+    }
+  }
 }
 
 int plan_alloc_workspace(Plan &p, size_t limbs, uint32_t **ws) {
@@ -193,6 +217,26 @@ int plan_run(const Plan &p, hipStream_t st) {
       case PlanStage::kBsFused:
         launch_bs_fused_table(c, s.d_bs, s.d_ew, (int)s.ew.size(), st);
         break;
+      case PlanStage::kNtt8Cols:
+        launch_ntt8_cols(c, p.i8, s.d_n8, (int)s.n8.size(), s.blocks, st);
+        break;
+      case PlanStage::kNtt8Rows:
+        launch_ntt8_rows(c, p.i8, s.d_n8, (int)s.n8.size(), s.blocks, st);
+        break;
+      case PlanStage::kBconvI8:
+      case PlanStage::kBconvValu: {
+        int dsts = 0, max_s = 1, max_d = 1;
+        for (const BconvI8Inst &I : s.bi) {
+          dsts += I.D;
+          max_s = std::max(max_s, I.S);
+          max_d = std::max(max_d, I.D);
+        }
+        if (s.kind == PlanStage::kBconvI8)
+          launch_bconv_i8_table(c, s.d_bi, (int)s.bi.size(), dsts, max_s, max_d, st);
+        else
+          launch_bconv_valu_table(c, s.d_bi, (int)s.bi.size(), dsts, max_s, st);
+        break;
+      }
       case PlanStage::kBconv: {
         int dsts = 0, max_s = 1, max_d = 1;
         for (const BconvInst &I : s.bc) {

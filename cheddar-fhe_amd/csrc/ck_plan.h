@@ -1,6 +1,7 @@
 // Batched plans (see ck_plan.cpp).
 #pragma once
 
+#include "ck_i8.h"
 #include "ck_ops.h"
 
 namespace ck {
@@ -8,7 +9,11 @@ namespace ck {
 // One step of a plan: a transform pass over every job's limb, or one
 // key-accumulation kernel over every item's limbs.
 struct PlanStage {
-  enum Kind { kFwd, kInv, kKeyAccum, kKeyRows, kBsFused, kBconv } kind = kFwd;
+  enum Kind {
+    kFwd, kInv, kKeyAccum, kKeyRows, kBsFused, kBconv,
+    kNtt8Cols, kNtt8Rows,   // int8 matrix-core transform passes
+    kBconvI8, kBconvValu    // BaseConv on the int8 matrix cores / vector unit
+  } kind = kFwd;
   int flags_any = 0;
   int max_bs = 0;  // largest BaseConv source count among the jobs
   std::vector<LimbJob> jobs;
@@ -16,8 +21,13 @@ struct PlanStage {
   std::vector<BsFusedItem> bs;
   std::vector<EwJob> ew;
   std::vector<BconvInst> bc;
+  std::vector<Ntt8Job> n8;      // kNtt8Cols / kNtt8Rows
+  std::vector<BconvI8Inst> bi;  // kBconvI8 / kBconvValu
+  int blocks = 16;              // int8 passes: 16-column / 16-row blocks per workgroup
   LimbJob *d_jobs = nullptr;
   BconvInst *d_bc = nullptr;
+  Ntt8Job *d_n8 = nullptr;
+  BconvI8Inst *d_bi = nullptr;
   KeyAccumItem *d_ka = nullptr;
   BsFusedItem *d_bs = nullptr;
   EwJob *d_ew = nullptr;
@@ -33,6 +43,15 @@ struct Plan {
   // autotuning record: measured run time of each BaseConv form (us; < 0:
   // not measured)
   double tuned_us[2] = {-1.0, -1.0};
+  // the form of a transform or BaseConv plan (ck_plan_info) and every
+  // candidate that autotuning timed: form, blocks, chunk, run time (us)
+  int form = 0;
+  struct Candidate {
+    int form, blocks, chunk;
+    double us;
+  };
+  std::vector<Candidate> tuning;
+  I8NttTabs i8{};
 };
 
 // Records the ops of one item after another; begin_item() rewinds to the
@@ -63,6 +82,13 @@ int plan_alloc_workspace(Plan &p, size_t limbs, uint32_t **ws);
 // a constant table the plan owns (freed by plan_destroy)
 int plan_upload(Plan &p, double **dst, const std::vector<double> &v);
 int plan_finish(Plan &p, bool mismatch);
+// Appends the int8 forward transform of `jobs` to p: column pass then row
+// pass per chunk of `chunk` limbs (0: all jobs in one chunk), so a chunk's
+// intermediate can stay in the last-level cache between its two passes;
+// jobs are ordered by prime first.
+void plan_add_ntt8(Plan &p, std::vector<Ntt8Job> jobs, int blocks, int chunk);
+// a constant int table the plan owns (freed by plan_destroy)
+int plan_upload_i64(Plan &p, int64_t **dst, const std::vector<int64_t> &v);
 int plan_run(const Plan &p, hipStream_t st);
 void plan_destroy(Plan *p);
 int plan_launches(const Plan &p);

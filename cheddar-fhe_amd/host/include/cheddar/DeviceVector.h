@@ -141,14 +141,18 @@ class DeviceVector {
   const word *data() const { return ptr_; }
 
   // Reallocates when the size changes; the leading min(old, new) words are
-  // kept.
+  // kept. The copy waits for all device work first: the old contents may
+  // still be written by kernels on any stream (a plain hipMemcpy is not
+  // ordered against non-blocking streams).
   void resize(size_t size) {
     if (size == size_) return;
     word *p = size ? (word *)DevicePool::Get().Alloc(size * sizeof(word)) : nullptr;
-    if (ptr_ && p && size_)
+    if (ptr_ && p && size_) {
+      HipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
       HipCheck(hipMemcpy(p, ptr_, std::min(size, size_) * sizeof(word),
                          hipMemcpyDeviceToDevice),
                "hipMemcpy");
+    }
     release();
     ptr_ = p;
     size_ = size;

@@ -27,6 +27,9 @@ class BootGraph {
     if (inputs.size() != outputs.size() || inputs.empty())
       throw Error("BootGraph: one output per input");
     if (!stream) throw Error("BootGraph: capture needs a non-default stream");
+    // the graph reuses pooled blocks that eager work on the context's own
+    // stream may still be using: let all device work finish first
+    HipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
     void *old = ctx.Stream();
     ctx.SetStream(stream);
     DevicePool::Get().BeginScope(&scope_);

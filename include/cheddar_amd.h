@@ -326,9 +326,19 @@ int ck_plan_ntt(ck_context *ctx, int n, int num_main, int num_ter, int num_aux,
                 uint32_t *const *dst, ck_plan **out);
 int ck_plan_run(ck_plan *plan, void *stream);
 /* out[8]: items, steps, kernel launches per run, workspace bytes, BaseConv
- * form (1: matrix cores, 0: fused), and with autotuning the measured run
- * time of each form in ns (fused, matrix cores; -1000 when not tuned), 0 */
+ * form of an HMult / hoisted plan (1: matrix cores, 0: fused), with
+ * CK_OPT_PLAN_AUTOTUNE the measured run time of each of those forms in ns
+ * (fused, matrix cores; -1000 when not tuned), and the form of a transform
+ * or BaseConv plan (CK_OPT_PLAN_NTT_FORM / CK_OPT_PLAN_BCONV_FORM values) */
 int ck_plan_info(const ck_plan *plan, long long *out);
+/* Autotuning record of a plan built with an autotuned form option: for up
+ * to max candidates, forms[3 i .. 3 i + 2] = (form, int8 blocks per
+ * workgroup, int8 chunk in limbs) and us[i] = its measured run time in
+ * microseconds. Returns the number of candidates (0: not autotuned), or -1
+ * on error. No reference counterpart (the reference fixes its launch
+ * shapes at compile time, include/core/NTTUtils.cuh:395-437,
+ * src/core/ModSwitch.cu:11-19). */
+int ck_plan_tuning(const ck_plan *plan, int max, int *forms, double *us);
 int ck_plan_destroy(ck_plan *plan);
 
 /* ---- Options ------------------------------------------------------------------ */
@@ -351,6 +361,28 @@ int ck_plan_destroy(ck_plan *plan);
  * 0 runs them as a row pass plus a separate accumulation. Results are
  * identical. */
 #define CK_OPT_FUSED_KEY_ROWS 3
+/* CK_OPT_PLAN_NTT_FORM: the form of forward transforms in ck_plan_ntt.
+ * 0 (default): radix-16 FP64 butterflies on the vector unit (NTTPhase1 /
+ * NTTPhase2, src/core/NTT.cu:231-454); 1: the 4-step 256 x 256 transform as
+ * exact int8 matrix products (residues split into four bytes, int32
+ * accumulation, exact recombination); -1: autotune, i.e. build the vector
+ * form and the int8 form for every (CK_OPT_NTT8_BLOCKS, CK_OPT_NTT8_CHUNK)
+ * candidate, time each and keep the fastest (ck_plan_tuning lists them;
+ * outputs must not alias inputs). Inverse transforms always use the vector
+ * form. Results are identical. */
+#define CK_OPT_PLAN_NTT_FORM 4
+/* CK_OPT_PLAN_BCONV_FORM: the form of ck_plan_bconv. 0 (default): FP64
+ * matrix cores (16-bit table halves); 1: int8 matrix cores (residues and
+ * table split into bytes); 2: vector unit (exact FP64 FMAs); -1: autotune
+ * over the three. Results are identical. */
+#define CK_OPT_PLAN_BCONV_FORM 5
+/* CK_OPT_NTT8_BLOCKS: 16-column (16-row) blocks of one limb per workgroup
+ * of the int8 transform's column (row) pass: 1, 2, 4, 8 or 16 (default). */
+#define CK_OPT_NTT8_BLOCKS 6
+/* CK_OPT_NTT8_CHUNK: limbs per chunk of an int8 transform plan, whose
+ * column and row passes run chunk by chunk so that a chunk's intermediate
+ * can stay in the last-level cache; 0 (default): one chunk. */
+#define CK_OPT_NTT8_CHUNK 7
 int ck_context_set_option(ck_context *ctx, int option, int value);
 
 /* ---- Per-kernel timing ---------------------------------------------------- */

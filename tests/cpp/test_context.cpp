@@ -22,6 +22,8 @@ using namespace cheddar;
 using word = uint32_t;
 using Ct = Ciphertext<word>;
 using Evk = EvaluationKey<word>;
+using Pt = Plaintext<word>;
+using Const = Constant<word>;
 using Host = std::vector<word>;
 
 namespace {
@@ -260,6 +262,40 @@ int main(int argc, char **argv) {
     Expect("MultKeyNoModDown bx", acc.bx_, wb);
     Expect("MultKeyNoModDown ax", acc.ax_, wa);
     Report("MultKeyNoModDown NP", acc.GetNP().num_aux_ == param->alpha_);
+  });
+
+  // in-place MultUnsafe of a Q*P ciphertext by a Q-only plaintext and by a
+  // Q-only constant: the output drops the aux limbs, so it must not reuse
+  // (and reallocate) the input's buffers before reading them
+  Run("In-place MultUnsafe of a Q*P ciphertext", [&] {
+    const size_t ts = (size_t)(np.GetNumQ() + param->alpha_) * n;
+    Host wb(ts), wa(ts);
+    or_mult_key_no_moddown(H.op, level, a1.data(), b1.data(), nullptr, mkb.data(),
+                           mka.data(), wb.data(), wa.data());
+    Pt pt(np);
+    pt.mx_.CopyFromHost(b2);
+    Const k(np);
+    Host kc(np.GetNumTotal());
+    for (size_t t = 0; t < kc.size(); t++) kc[t] = a2[t * n + 7];  // any residues
+    k.cx_.CopyFromHost(kc);
+    Host pb(qs), pa(qs), cb(qs), ca(qs), kfull(qs);
+    for (size_t t = 0; t < kc.size(); t++)
+      std::fill(kfull.begin() + t * n, kfull.begin() + (t + 1) * n, kc[t]);
+    or_mult(H.op, pidx.data(), (int)pidx.size(), wb.data(), b2.data(), pb.data());
+    or_mult(H.op, pidx.data(), (int)pidx.size(), wa.data(), b2.data(), pa.data());
+    or_mult(H.op, pidx.data(), (int)pidx.size(), wb.data(), kfull.data(), cb.data());
+    or_mult(H.op, pidx.data(), (int)pidx.size(), wa.data(), kfull.data(), ca.data());
+    Ct acc;
+    ctx->MultKeyNoModDown(acc, c1, mk);
+    ctx->MultUnsafe(acc, acc, pt);
+    Report("in-place Ct*Pt NP", acc.GetNP() == np);
+    Expect("in-place Ct*Pt bx", acc.bx_, pb);
+    Expect("in-place Ct*Pt ax", acc.ax_, pa);
+    ctx->MultKeyNoModDown(acc, c1, mk);
+    ctx->MultUnsafe(acc, acc, k);
+    Report("in-place Ct*Const NP", acc.GetNP() == np);
+    Expect("in-place Ct*Const bx", acc.bx_, cb);
+    Expect("in-place Ct*Const ax", acc.ax_, ca);
   });
 
   std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);

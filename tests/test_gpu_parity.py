@@ -424,6 +424,21 @@ def test_permute_accum(ck, ctx24, oracle_rns24, accumulate):
         assert_same(ck.to_host(outs[j]), want, f"permute_accum poly {j}")
 
 
+@pytest.mark.parametrize("accumulate", [False, True])
+def test_permute_accum_no_sources(ck, ctx24, oracle_rns24, accumulate):
+    """an empty sum of automorphisms over all 32 limbs: zeros, or dst
+    unchanged when accumulating (the masked terms gather in bounds)"""
+    P = oracle_rns24
+    pidx = P.np_primes(24, 0, 8)
+    init = [uniform(P, pidx, SEED_BASE + 660 + j) for j in range(2)]
+    outs = [ck.to_device(v) for v in init]
+    ctx24.permute_accum(outs, [], [], (24, 0, 8), accumulate=accumulate)
+    sync()
+    for j in range(2):
+        want = init[j] if accumulate else np.zeros_like(init[j])
+        assert_same(ck.to_host(outs[j]), want, f"empty permute_accum poly {j}")
+
+
 # ---------------------------------------------------------------------------
 # Context evaluator
 # ---------------------------------------------------------------------------
