@@ -42,7 +42,8 @@
 // leave a stage's output index where the next stage needs its input index).
 // The column pass reads canonical input as x - 2^30 (fits the digit range)
 // and adds 2^30 * rowsum(A1) back through the accumulator's initial value.
-// Intermediate values between the passes are centred int32 in dst.
+// Intermediate values between the passes are centred int32 in dst, in
+// 16 x 16 tiles (tile_word).
 #include "ck_i8.h"
 #include "ck_i8_arith.h"
 
@@ -75,11 +76,38 @@ __device__ __forceinline__ v4i mfma64(v4i a, v4i b, v4i c) {
 }
 
 typedef const __attribute__((address_space(4))) Ntt8Job CJob;
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+
+// data streams through once: non-temporal accesses keep it from evicting the
+// twiddle table and operand matrices, which every limb of a prime re-reads
+__device__ __forceinline__ uint32_t ld_nt(const uint32_t *p) {
+  return __builtin_nontemporal_load(p);
+}
+__device__ __forceinline__ v4u ld4_nt(const uint32_t *p) {
+  return __builtin_nontemporal_load((const v4u *)p);
+}
+__device__ __forceinline__ void st_nt(uint32_t *p, uint32_t v) {
+  __builtin_nontemporal_store(v, p);
+}
+__device__ __forceinline__ void st4_nt(uint32_t *p, v4u v) {
+  __builtin_nontemporal_store(v, (v4u *)p);
+}
+
+// Intermediate layout between the passes: 16 x 16 tiles of 1 KiB, tile
+// (p1, cb) = rows 16 p1 .. 16 p1 + 15, columns 16 cb .. 16 cb + 15 at word
+// (16 p1 + cb) * 256; inside a tile, row p0 = 4 g + v and column n sit at
+// word (16 g + n) * 4 + v. The column pass writes a tile with one 16-byte
+// store per lane (1 KiB per wave), the row pass of rows 16 p1.. reads the
+// 16 KiB of its 16 tiles contiguously.
+__device__ __forceinline__ int tile_word(int p1, int cb, int g, int n) {
+  return (16 * p1 + cb) * 256 + (16 * g + n) * 4;
+}
 
 // Column pass: workgroup = one limb's 16-column blocks [part * cbs, +cbs).
 // Wave w runs the first stage for r0 = 4w..4w+3 and the second stage for
-// p1 = 4w..4w+3, so it keeps A1 and four B1 matrices in registers.
-__global__ __launch_bounds__(256) void ntt8_cols_kernel(const Ntt8Job *__restrict__ jobs,
+// p1 = 4w..4w+3, so it keeps A1 and four B1 matrices in registers. The
+// next block's inputs are loaded while the current one is transformed.
+__global__ __launch_bounds__(256, 3) void ntt8_cols_kernel(const Ntt8Job *__restrict__ jobs,
                                                         int nwg, int cbs, I8NttTabs T,
                                                         PrimeTab P) {
   __shared__ __align__(16) int xs[2][16 * kXs];
@@ -108,14 +136,25 @@ __global__ __launch_bounds__(256) void ntt8_cols_kernel(const Ntt8Job *__restric
   const double q = P.qd[pr], qr = P.qr[pr];
   const uint32_t *__restrict__ tw = T.tw + (size_t)pr * kN;
 
-  for (int cb = part * cbs; cb < part * cbs + cbs; cb++) {
+  // lane's first input word of a block: row 4w + 64 g, column n
+  const uint32_t *in0 = src + (4 * w + 64 * g) * 256 + n;
+  const int cb0 = part * cbs, cb1 = cb0 + cbs;
+  uint32_t raw[4][4];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int t = 0; t < 4; t++) raw[k][t] = ld_nt(in0 + (k + 16 * t) * 256 + cb0 * 16);
+
+  for (int cb = cb0; cb < cb1; cb++) {
     int *X = xs[cb & 1];
     const int col = cb * 16 + n;
-    uint32_t raw[4][4];
+    // next block's inputs (the last iteration reloads its own block)
+    const int cbn = cb + 1 < cb1 ? cb + 1 : cb;
+    uint32_t nxt[4][4];
 #pragma unroll
     for (int k = 0; k < 4; k++)
 #pragma unroll
-      for (int t = 0; t < 4; t++) raw[k][t] = src[(4 * w + k + 16 * (4 * g + t)) * 256 + col];
+      for (int t = 0; t < 4; t++) nxt[k][t] = ld_nt(in0 + (k + 16 * t) * 256 + cbn * 16);
     // first stage: 16-point transforms along r1, tile r0 = 4w + k
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -131,13 +170,14 @@ __global__ __launch_bounds__(256) void ntt8_cols_kernel(const Ntt8Job *__restric
         X[(4 * g + v) * kXs + (4 * w + k) * 20 + n] = pack_c((int)r);
       }
     }
+    __syncthreads();
     uint32_t twv[4][4];
 #pragma unroll
     for (int k = 0; k < 4; k++)
 #pragma unroll
       for (int v = 0; v < 4; v++) twv[k][v] = tw[(4 * g + v + 16 * (4 * w + k)) * 256 + col];
-    __syncthreads();
-    // second stage along r0 (tile p1 = 4w + k), then the 4-step twiddle
+    // second stage along r0 (tile p1 = 4w + k), then the 4-step twiddle;
+    // one 16-byte store per lane writes the tile (p1, cb)
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       const int p1 = 4 * w + k;
@@ -147,22 +187,28 @@ __global__ __launch_bounds__(256) void ntt8_cols_kernel(const Ntt8Job *__restric
       const v4i d1 = mfma64(b1[k][1], b, zero);
       const v4i d2 = mfma64(b1[k][2], b, zero);
       const v4i d3 = mfma64(b1[k][3], b, zero);
+      v4u out;
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int ph = 4 * g + v + 16 * p1;
         const double r = reduce_q(combine(d0[v], d1[v], d2[v], d3[v]), q, qr);
         const double z = mulmod_fq(r, (double)twv[k][v], q, qr);
-        dst[ph * 256 + col] = (uint32_t)(int)z;
+        out[v] = (uint32_t)(int)z;
       }
+      st4_nt(dst + tile_word(p1, cb, g, n), out);
     }
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+      for (int t = 0; t < 4; t++) raw[k][t] = nxt[k][t];
   }
 }
 
 // Row pass (in place on dst): workgroup = one limb's 16-row blocks
 // [part * rbs, +rbs). Wave w runs the first stage for rows 16 rb + 4w + k
-// and the second stage for q1 = 4w + k with the data as the A operand, so
-// each lane ends with four rows of one output column (16-lane runs of
-// consecutive outputs).
+// (one 16-byte load per lane and tile gives its four rows) and the second
+// stage for q1 = 4w + k with the data as the A operand, so each lane ends
+// with four rows of one output column. The next block's tiles are loaded
+// while the current one is transformed.
 __global__ __launch_bounds__(256) void ntt8_rows_kernel(const Ntt8Job *__restrict__ jobs,
                                                         int nwg, int rbs, I8NttTabs T,
                                                         PrimeTab P) {
@@ -187,22 +233,23 @@ __global__ __launch_bounds__(256) void ntt8_rows_kernel(const Ntt8Job *__restric
   const v4i zero = {0, 0, 0, 0};
   const double q = P.qd[pr], qr = P.qr[pr];
 
-  for (int rb = part * rbs; rb < part * rbs + rbs; rb++) {
+  const int rb0 = part * rbs, rb1 = rb0 + rbs;
+  // tile (rb, c1 = 4g + t), rows 4w + k (k = 0..3 in one load), column c0 = n
+  v4u raw[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) raw[t] = ld4_nt(Z + tile_word(rb0, 4 * g + t, w, n));
+
+  for (int rb = rb0; rb < rb1; rb++) {
     int *W = xs[rb & 1];
-    int raw[4][4];
+    const int rbn = rb + 1 < rb1 ? rb + 1 : rb;
+    v4u nxt[4];
 #pragma unroll
-    for (int k = 0; k < 4; k++)
-#pragma unroll
-      for (int t = 0; t < 4; t++)
-        raw[k][t] = (int)Z[(16 * rb + 4 * w + k) * 256 + n + 16 * (g + 4 * t)];
-    // first stage along c1 (row 16 rb + 4w + k): V[q1 = 4g + v][c0 = n];
-    // register t of lane group g holds c1 = g + 4t (the A3 operand is
-    // built in that order), so each load instruction reads 256
-    // consecutive bytes of the row
+    for (int t = 0; t < 4; t++) nxt[t] = ld4_nt(Z + tile_word(rbn, 4 * g + t, w, n));
+    // first stage along c1 (row 16 rb + 4w + k): V[q1 = 4g + v][c0 = n]
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const v4i b = {pack_c(raw[k][0]), pack_c(raw[k][1]), pack_c(raw[k][2]),
-                     pack_c(raw[k][3])};
+      const v4i b = {pack_c((int)raw[0][k]), pack_c((int)raw[1][k]), pack_c((int)raw[2][k]),
+                     pack_c((int)raw[3][k])};
       const v4i d0 = mfma64(a3[0], b, zero);
       const v4i d1 = mfma64(a3[1], b, zero);
       const v4i d2 = mfma64(a3[2], b, zero);
@@ -214,6 +261,7 @@ __global__ __launch_bounds__(256) void ntt8_rows_kernel(const Ntt8Job *__restric
       }
     }
     __syncthreads();
+    // the row block's tiles are all read: the outputs may overwrite them
     // second stage along c0 (q1 = 4w + k): rows as A (row n, c0 = 4g + t)
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -227,9 +275,11 @@ __global__ __launch_bounds__(256) void ntt8_rows_kernel(const Ntt8Job *__restric
       for (int v = 0; v < 4; v++) {
         const int row = 16 * rb + 4 * g + v;
         const double r = reduce_q(combine(d0[v], d1[v], d2[v], d3[v]), q, qr);
-        Z[row * 256 + n + 16 * q1] = canon_small(r, q);
+        st_nt(Z + row * 256 + n + 16 * q1, canon_small(r, q));
       }
     }
+#pragma unroll
+    for (int t = 0; t < 4; t++) raw[t] = nxt[t];
   }
 }
 
@@ -400,7 +450,7 @@ int ntt8_tables(Context &c, I8NttTabs *out) {
     }
     for (int q1 = 0; q1 < 16; q1++)
       for (int c1 = 0; c1 < 16; c1++) T[q1][c1] = h_powmod(om, 16ull * c1 * brv_n(q1, 4), q);
-    i8::ctx_planes(T, q, pp + (size_t)kI8CtxA3 * 1024, true);
+    i8::ctx_planes(T, q, pp + (size_t)kI8CtxA3 * 1024);
     for (int q1 = 0; q1 < 16; q1++) {
       for (int q0 = 0; q0 < 16; q0++)
         for (int c0 = 0; c0 < 16; c0++)

@@ -20,12 +20,22 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
 # minimum waves per SIMD for the transform passes (LDS allows five); the
 # column pass's instantiation for BaseConv over more than 9 sources (mangled
 # with template argument Lb1E: WIDE = true) is built for four
-MIN_OCCUPANCY = {"ntt_fwd_cols": 5, "ntt_fwd_rows": 5, "intt_rows": 5, "intt_cols": 5}
+MIN_OCCUPANCY = {"ntt_fwd_cols": 5, "ntt_fwd_rows": 5, "intt_rows": 5, "intt_cols": 5,
+                 "ntt8_cols": 3, "ntt8_rows": 3}
 WIDE_OCCUPANCY = 4
 
 
+def extra_flags(src):
+    """the per-file flags the build adds (cheddar-fhe_amd/build.py EXTRA)"""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("ck_build", os.path.join(PKG, "build.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.EXTRA.get(src, [])
+
+
 def resources(src):
-    r = subprocess.run([HIPCC] + FLAGS + [os.path.join(CSRC, src)],
+    r = subprocess.run([HIPCC] + FLAGS + extra_flags(src) + [os.path.join(CSRC, src)],
                        capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-2000:]
     out, cur = {}, None
@@ -43,7 +53,7 @@ def resources(src):
 
 
 @pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
-@pytest.mark.parametrize("src", ["ck_ntt.hip", "ck_elem.hip", "ck_bconv.hip"])
+@pytest.mark.parametrize("src", ["ck_ntt.hip", "ck_elem.hip", "ck_bconv.hip", "ck_i8.hip"])
 def test_no_spills_and_occupancy(src):
     res = resources(src)
     assert res, "no kernels reported"
