@@ -89,6 +89,9 @@ _SIGS = {
                                       ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, _vp]),
     "ck_hmult": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
                                 ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, _vp]),
+    "ck_hmult_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                        ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int,
+                                        _vp]),
     "ck_hrot": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp,
                                ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, _vp]),
     "ck_hconj": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp,
@@ -378,6 +381,15 @@ class Context:
         _check(lib.ck_hmult(self._h, level, _ptr(out[0]), _ptr(out[1]), _ptr(ct1[0]),
                             _ptr(ct1[1]), _ptr(ct2[0]), _ptr(ct2[1]), _ptrs(key[0]),
                             _ptrs(key[1]), int(rescale), _stream(stream)))
+
+    def hmult_noalloc(self, out, ct1, ct2, key, level, rescale=True, stream=None):
+        """hmult with its temporaries in the library's static device array
+        (ck_hmult_noalloc: allocates nothing; calls on one device must not
+        overlap)"""
+        _check(lib.ck_hmult_noalloc(self._h, level, _ptr(out[0]), _ptr(out[1]),
+                                    _ptr(ct1[0]), _ptr(ct1[1]), _ptr(ct2[0]), _ptr(ct2[1]),
+                                    _ptrs(key[0]), _ptrs(key[1]), int(rescale),
+                                    _stream(stream)))
 
     def hrot(self, out, ct, key, rot, level, stream=None):
         _check(lib.ck_hrot(self._h, level, _ptr(out[0]), _ptr(out[1]), _ptr(ct[0]), _ptr(ct[1]),

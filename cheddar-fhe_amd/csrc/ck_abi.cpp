@@ -5,9 +5,23 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "ck_plan.h"
+
+// Workspace of ck_hmult_noalloc: a file-scope device array (192 limbs of
+// N = 2^16 words, 48 MiB). Every device the library runs on holds its own
+// copy from module load until the process ends. Host code never uses the
+// array's name as an address: scratch_ptr() returns the device address that
+// hipGetSymbolAddress reports for the current device.
+namespace {
+constexpr size_t kScratchLimbs = 192;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+static __device__ __align__(256) uint32_t scratch[kScratchLimbs * 65536];
 
 struct ck_context {
   ck::Context *c;
@@ -34,6 +48,27 @@ int fail(int code, const char *msg) {
   } while (0)
 
 hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept
+int scratch_ptr(uint32_t **out) {
+  *out = nullptr;
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail(CK_ERR_HIP, "no current HIP device for the static scratch");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return CK_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<uint32_t *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return CK_OK;
+}
 
 bool valid_np(const Context &c, int nm, int nt, int na) {
   return nm >= 0 && nm <= c.max_main && nt >= 0 && nt <= c.max_ter &&
@@ -102,6 +137,13 @@ int ck_context_create(int log_degree, const uint32_t *main_primes, int n_main,
   int rc = context_create(log_degree, main_primes, n_main, ter_primes, n_ter,
                           aux_primes, n_aux, level_config, n_levels, &c);
   if (rc) return rc;
+  // set-up of ck_hmult_noalloc's static scratch on this device
+  uint32_t *sp = nullptr;
+  rc = scratch_ptr(&sp);
+  if (rc) {
+    context_destroy(c);
+    return rc;
+  }
   *out = new ck_context{c};
   return CK_OK;
 }
@@ -583,6 +625,32 @@ int ck_hmult(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
   ImmediateSink sink(c, as_stream(stream));
   return op_hmult(c, level, out_b, out_a, b1, a1, b2, a2, key_b, key_a,
                   rescale != 0, c.ws, sink);
+}
+
+int ck_hmult_noalloc(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
+                     const uint32_t *b1, const uint32_t *a1, const uint32_t *b2,
+                     const uint32_t *a2, const uint32_t *const *key_b,
+                     const uint32_t *const *key_a, int rescale, void *stream) {
+  CK_REQUIRE(ctx && ctx->c && out_b && out_a && b1 && a1 && b2 && a2,
+             "null argument");
+  const Context &c = *ctx->c;
+  CK_REQUIRE(valid_level(c, level), "invalid level");
+  CK_REQUIRE(!rescale || moddown_plan(c, level, 2),
+             "cannot rescale at this level");
+  CK_REQUIRE(valid_key(c, key_b, level) && valid_key(c, key_a, level),
+             "null key block");
+  CK_REQUIRE(relin_ws_limbs(c, level) <= kScratchLimbs,
+             "HMult at this level needs more workspace than the static scratch "
+             "holds (192 limbs of 2^16 words)");
+  CK_REQUIRE(c.n == 65536, "the static scratch is sized for N = 2^16");
+  // per-kernel profiling creates HIP events, i.e. allocates
+  CK_REQUIRE(!c.prof_on, "disable profiling (ck_profile_enable) for ck_hmult_noalloc");
+  uint32_t *ws = nullptr;
+  const int rc = scratch_ptr(&ws);
+  if (rc) return rc;
+  ImmediateSink sink(c, as_stream(stream));
+  return op_hmult(c, level, out_b, out_a, b1, a1, b2, a2, key_b, key_a,
+                  rescale != 0, ws, sink);
 }
 
 int ck_hrot(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,

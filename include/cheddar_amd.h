@@ -252,6 +252,20 @@ int ck_hmult(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
              const uint32_t *b1, const uint32_t *a1, const uint32_t *b2,
              const uint32_t *a2, const uint32_t *const *key_b,
              const uint32_t *const *key_a, int rescale, void *stream);
+/* ck_hmult without allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). No counterpart in the reference. Identical arguments
+ * and outputs to ck_hmult; its temporaries live in a static device array of
+ * 192 limbs of N = 2^16 words (48 MiB per device, held until the process
+ * ends) instead of the context's workspace, so it calls no allocator. Levels
+ * whose HMult needs more than that (relinearization workspace above 192
+ * limbs) are rejected with CK_ERR_INVALID_ARG, as is a call with
+ * per-kernel profiling on. Every call on one device shares that array:
+ * calls that overlap on one device (several streams or host threads) race. */
+int ck_hmult_noalloc(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
+                     const uint32_t *b1, const uint32_t *a1, const uint32_t *b2,
+                     const uint32_t *a2, const uint32_t *const *key_b,
+                     const uint32_t *const *key_a, int rescale, void *stream);
 /* Context::HRot (src/core/Context.cpp:776-791): MultKey then Permute by
  * 5^rot; rot is reduced mod N/2 (num_slots), rot == 0 copies. */
 int ck_hrot(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
