@@ -201,15 +201,35 @@ def copy_calibration(torch):
     log("copy calibration: 268435456 bytes copied")
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    """threads for the all-cores leg: the cores this process may run on,
+    at most 16 (the host CPU share of one GPU on the measurement boxes)"""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def cpu_baseline(prm, seconds=10.0):
-    """The CPU oracle's HMult at the same shape on the host cores (bounded
-    sample: as many HMults as fit in about `seconds`, at least 2)."""
+    """The CPU oracle's HMult at the same shape on the host cores, once on
+    one thread and once on every host thread (bounded samples: as many
+    HMults as fit in about `seconds` each, at least 2)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import numpy as np
     import ckks_oracle as O
 
-    cores = min(16, os.cpu_count() or 1)
-    O.set_threads(cores)
     P = O.Param.from_json(prm)
     pidx = P.level_primes(LEVEL)
     seed = 0x4348454444415200 + 1
@@ -218,36 +238,46 @@ def cpu_baseline(prm, seconds=10.0):
     allp = list(range(P.num_primes))
     key = (np.stack([P.uniform(allp, seed + 10 + g) for g in range(P.dnum)]),
            np.stack([P.uniform(allp, seed + 20 + g) for g in range(P.dnum)]))
-    P.hmult(LEVEL, ct1, ct2, key)  # warm-up
-    t0 = time.perf_counter()
-    n = 0
-    while True:
-        P.hmult(LEVEL, ct1, ct2, key)
-        n += 1
-        el = time.perf_counter() - t0
-        if (el >= seconds and n >= 2) or el >= 3 * seconds:
-            break
-    return {"value": n / el, "unit": "HMult/s", "cores": cores, "kind": "port",
-            "sample": f"{n} HMult+relin+rescale at L=24 on {cores} host threads "
-                      f"({el:.1f} s)"}
+
+    def rate(threads):
+        O.set_threads(threads)
+        P.hmult(LEVEL, ct1, ct2, key)  # warm-up
+        t0 = time.perf_counter()
+        n = 0
+        while True:
+            P.hmult(LEVEL, ct1, ct2, key)
+            n += 1
+            el = time.perf_counter() - t0
+            if (el >= seconds and n >= 2) or el >= 3 * seconds:
+                return n / el, n, el
+
+    cores = host_threads()
+    v1, n1, e1 = rate(1)
+    vn, nn, en = rate(cores)
+    return {"value": vn, "unit": "HMult/s", "cores": cores, "kind": "port",
+            "sample": f"{nn} HMult+relin+rescale at L=24 on {cores} host threads "
+                      f"({en:.1f} s); {n1} on 1 thread ({e1:.1f} s)",
+            "value_1_thread": v1, "nproc": os.cpu_count(), "cpu_model": cpu_model()}
 
 
 BATCH = 64
 BATCH_ROTS = 16
 
 
-def bench_batched(a, torch, ck):
-    """Batched key switching: BATCH ciphertexts with limb counts uniform in
-    [8, 24] (seeded) each go through HMult+relin+rescale (one plan, one
-    launch per step across the batch), then BATCH_ROTS hoisted rotations by
-    distinct offsets in [1, 2^15) of each product (a second plan sharing one
-    set of rotation keys). Steps = runs of both plans; HMult/s and HRot/s
-    are whole-batch throughputs."""
+def run_batched(torch, ck, steps, warmup, autotune=True, bconv="auto", check=4):
+    """Batched key switching (config 2): BATCH ciphertexts with limb counts
+    uniform in [8, 24] (seeded) each go through HMult+relin+rescale (one
+    plan, one launch per step across the batch), then BATCH_ROTS hoisted
+    rotations by distinct offsets in [1, 2^15) of each product (a second
+    plan sharing one set of rotation keys). Returns whole-batch HMult/s and
+    HRot/s. Outside the timed region, `check` seeded items of each plan are
+    compared word for word with the single-ciphertext entry points
+    (ck_hmult, ck_hrot) on the same inputs."""
     import random
     prm = ck.load_param_json(PARAMS)
     ctx = ck.Context.from_json(prm)
-    ctx.set_bconv_mfma(None if a.bconv == "auto" else a.bconv == "mfma")
-    ctx.set_plan_autotune(not a.no_autotune)
+    ctx.set_bconv_mfma(None if bconv == "auto" else bconv == "mfma")
+    ctx.set_plan_autotune(autotune)
     torch.cuda.set_device(0)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(0x4348454444415202)
@@ -278,25 +308,41 @@ def bench_batched(a, torch, ck):
     log(f"plans built in {time.perf_counter() - t0:.1f} s: hmult {p_mult.info()}, "
         f"hoisted {p_rot.info()}")
     stream = torch.cuda.current_stream()
-    for _ in range(a.warmup):
+    for _ in range(warmup):
         p_mult.run(stream)
         p_rot.run(stream)
     torch.cuda.synchronize()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     ev[0].record(stream)
-    for _ in range(a.steps):
+    for _ in range(steps):
         p_mult.run(stream)
     ev[1].record(stream)
-    for _ in range(a.steps):
+    for _ in range(steps):
         p_rot.run(stream)
     ev[2].record(stream)
     torch.cuda.synchronize()
-    t_mult = ev[0].elapsed_time(ev[1]) / a.steps
-    t_rot = ev[1].elapsed_time(ev[2]) / a.steps
+    t_mult = ev[0].elapsed_time(ev[1]) / steps
+    t_rot = ev[1].elapsed_time(ev[2]) / steps
     hmult_s = BATCH / (t_mult * 1e-3)
     hrot_s = BATCH * BATCH_ROTS / (t_rot * 1e-3)
     log(f"batched HMult: {t_mult:.3f} ms per {BATCH} ({hmult_s:.0f}/s); hoisted HRot: "
         f"{t_rot:.3f} ms per {BATCH * BATCH_ROTS} ({hrot_s:.0f}/s)")
+    # sampled items against the single-ciphertext entry points
+    sample = rng.sample(range(BATCH), check)
+    ok = True
+    for i in sample:
+        lv = levels[i]
+        o = [torch.empty_like(t) for t in prod[i]]
+        ctx.hmult(o, ct1[i], ct2[i], mkey, lv, rescale=True, stream=stream)
+        r = rng.randrange(BATCH_ROTS)
+        orot = [torch.empty_like(t) for t in rot_out[i][r]]
+        ctx.hrot(orot, prod[i], rkeys[r], rots[r], lv - 1, stream=stream)
+        torch.cuda.synchronize()
+        ok = ok and all(torch.equal(x, y) for x, y in zip(o, prod[i]))
+        ok = ok and all(torch.equal(x, y) for x, y in zip(orot, rot_out[i][r]))
+    log(f"batched outputs of items {sample} equal to ck_hmult / ck_hrot: {ok}")
+    if not ok:
+        raise RuntimeError("batched plan output differs from the single-ciphertext path")
     # per-kernel durations of one run of each plan (library dispatch events)
     for name, plan in (("hmult", p_mult), ("hoisted", p_rot)):
         ctx.profile(True)
@@ -308,19 +354,31 @@ def bench_batched(a, torch, ck):
         for k, (ms, calls) in sorted(rep.items(), key=lambda kv: -kv[1][0]):
             log(f"  {name} plan: {k:32s} {calls} x {1e3 * ms / calls:8.1f} us "
                 f"({100 * ms / tot:4.1f}%)")
+    return {"workload": f"{BATCH} ciphertexts at 8..24 limbs: HMult+relin+rescale plan, "
+                        f"then {BATCH_ROTS} hoisted HRot each",
+            "hmult_per_s": hmult_s, "hrot_per_s": hrot_s,
+            "ms_hmult_batch": t_mult, "ms_hrot_batch": t_rot, "steps": steps,
+            "levels": levels, "rotations": rots, "autotune": autotune,
+            "hmult_plan": p_mult.info(), "hrot_plan": p_rot.info(),
+            "hmult_plan_bconv_us": {("fused", "fp64", "int8")[f]: us
+                                    for f, _, _, us in p_mult.tuning()},
+            "hrot_plan_bconv_us": {("fused", "fp64", "int8")[f]: us
+                                   for f, _, _, us in p_rot.tuning()},
+            "checked_items": sample, "equal_to_single_ops": ok}
+
+
+def bench_batched(a, torch, ck):
+    """--workload batched: the batched configuration as its own JSON line"""
+    r = run_batched(torch, ck, a.steps, a.warmup, not a.no_autotune, a.bconv)
     res = {
-        "metric": "hmult_relin_rescale_per_s", "value": hmult_s, "unit": "HMult/s",
+        "metric": "hmult_relin_rescale_per_s", "value": r["hmult_per_s"], "unit": "HMult/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": t_mult + t_rot, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u32",
+        "ms_per_step": r["ms_hmult_batch"] + r["ms_hrot_batch"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32",
         "data": "synthetic uniform residues and keys (seeded)",
-        "config": {"workload": f"batched: {BATCH} ciphertexts at 8..24 limbs, HMult+relin+"
-                               f"rescale plan then {BATCH_ROTS} hoisted HRot each",
-                   "levels": levels, "rotations": rots, "log_n": 16, "alpha": ctx.alpha,
-                   "dnum": ctx.dnum, "parallelism": "single GPU",
-                   "autotune": not a.no_autotune,
-                   "hmult_plan": p_mult.info(), "hrot_plan": p_rot.info()},
-        "hrot_per_s": hrot_s, "ms_hmult_batch": t_mult, "ms_hrot_batch": t_rot,
+        "config": {"workload": "batched: " + r["workload"], "log_n": 16,
+                   "parallelism": "single GPU"},
+        "batched": r,
     }
     print(json.dumps(res), flush=True)
 
@@ -348,6 +406,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched workload: use --bconv instead of timing both BaseConv "
                          "forms per plan")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-plan measurement in the default run")
     ap.add_argument("--no-boot", action="store_true",
                     help="skip the bootstrapping measurement (bootparam_40, full slots)")
     ap.add_argument("--boot-runs", type=int, default=3)
@@ -443,6 +503,10 @@ def main():
         log(f"cpu baseline: {cpu['value']:.3f} HMult/s")
 
     config3 = bench_config3(torch, ck, ctx)
+    batched = None
+    if not a.no_batched:
+        batched = run_batched(torch, ck, steps=10, warmup=2)
+        batched["hmult_speedup_vs_plain"] = batched["hmult_per_s"] / (1e3 / ms)
     boot = None if a.no_boot else bench_boot(a.boot_runs)
 
     res = {
@@ -467,17 +531,24 @@ def main():
                      "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "config3": config3,
+        "batched": batched,
         "bootstrap": boot,
     }
     print(json.dumps(res), flush=True)
 
 
+INT8_MFMA_PEAK_TOPS = 5000.0  # MI355X dense int8 matrix peak
+
+
 def bench_config3(torch, ck, ctx, reps=10):
     """The config-3 contractions at their stated shape, 64 polynomials x 32
-    limbs, each as one batched plan: BaseConv 8 -> 32 limbs (the special
-    primes to all primes) on the FP64 matrix cores, and the forward NTT of
-    all 32 limbs of each polynomial. Inputs resident in HBM; HIP events on
-    the launch stream."""
+    limbs, each as one batched plan in every form: BaseConv 8 -> 32 limbs
+    (the special primes to all primes) on the int8 matrix cores, on the FP64
+    matrix cores and on the vector unit; the forward NTT of all 32 limbs of
+    each polynomial as the int8 4-step transform and as radix-16 FP64
+    butterflies on the vector unit, and the autotuned transform plan. All
+    forms' outputs are compared on the GPU (they must be identical). Inputs
+    resident in HBM; HIP events on the launch stream."""
     npoly, src_p, dst_p = 64, list(range(24, 32)), list(range(32))
     gen = torch.Generator(device="cuda")
     gen.manual_seed(0x4348454444415203)
@@ -489,9 +560,6 @@ def bench_config3(torch, ck, ctx, reps=10):
             v[i] = torch.randint(-h, h + 1, (ctx.n,), generator=gen, device="cuda",
                                  dtype=torch.int64).to(torch.int32)
         srcs.append(v)
-    dsts = [torch.empty((len(dst_p), ctx.n), dtype=torch.int32, device="cuda")
-            for _ in range(npoly)]
-    plan = ck.Plan.bconv(ctx, src_p, dst_p, srcs, dsts)
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
@@ -505,29 +573,77 @@ def bench_config3(torch, ck, ctx, reps=10):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps  # ms
 
-    ms_b = timed(lambda: plan.run(stream))
     b_bytes = npoly * (len(src_p) + len(dst_p)) * ctx.n * 4
-    b_flop = npoly * len(src_p) * len(dst_p) * ctx.n * 2 * 2   # {hi, lo} FP64 MACs
+    # matrix-core work as executed: int8 16x16x32 steps over K = 8 sources x
+    # 4 digits, four byte planes; FP64 16x16x4 over {hi, lo} table halves
+    b_int8_ops = npoly * len(dst_p) * ctx.n * 32 * 4 * 2
+    b_f64_flop = npoly * len(src_p) * len(dst_p) * ctx.n * 2 * 2
+    bconv, outs = {}, {}
+    for form in ("int8", "fp64", "valu"):
+        ctx.set_plan_forms(bconv=form)
+        d = [torch.empty((len(dst_p), ctx.n), dtype=torch.int32, device="cuda")
+             for _ in range(npoly)]
+        plan = ck.Plan.bconv(ctx, src_p, dst_p, srcs, d)
+        t = timed(lambda: plan.run(stream))
+        outs[form] = d
+        rec = {"ms": t, "hbm_gb_s": b_bytes / (t * 1e-3) / 1e9,
+               "hbm_frac": b_bytes / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        if form == "int8":
+            rec["int8_mfma_tops"] = b_int8_ops / (t * 1e-3) / 1e12
+            rec["int8_mfma_frac"] = rec["int8_mfma_tops"] / INT8_MFMA_PEAK_TOPS
+        if form == "fp64":
+            rec["fp64_mfma_tflop_s"] = b_f64_flop / (t * 1e-3) / 1e12
+            rec["fp64_mfma_frac"] = rec["fp64_mfma_tflop_s"] / FP64_MFMA_PEAK_TFLOPS
+        bconv[form] = rec
+    bconv_equal = all(torch.equal(x, y) for f in ("fp64", "valu")
+                      for x, y in zip(outs[f], outs["int8"]))
+    ctx.set_plan_forms(bconv="int8")
+    del outs
+
     polys = [uniform_poly(torch, ctx, dst_p, gen) for _ in range(npoly)]
-    outs = [torch.empty_like(x) for x in polys]
-    nplan = ck.Plan.ntt(ctx, polys, outs, (ctx.L, 0, ctx.alpha))
-    ms_n = timed(lambda: nplan.run(stream))
     n_bytes = npoly * len(dst_p) * ctx.n * 8
+    # int8 transform: four 16-point stages, each 4 planes of K = 16 x 4 digits
+    n_int8_ops = npoly * len(dst_p) * ctx.n * 4 * 4 * 64 * 2
+    ntt, nouts = {}, {}
+    for form in ("int8", "valu", "auto"):
+        ctx.set_plan_forms(ntt=form)
+        o = [torch.empty_like(x) for x in polys]
+        plan = ck.Plan.ntt(ctx, polys, o, (ctx.L, 0, ctx.alpha))
+        t = timed(lambda: plan.run(stream))
+        nouts[form] = o
+        rec = {"ms": t, "us_per_limb": 1e3 * t / (npoly * len(dst_p)),
+               "hbm_gb_s": n_bytes / (t * 1e-3) / 1e9,
+               "hbm_frac": n_bytes / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        if form == "int8":
+            rec["int8_mfma_tops"] = n_int8_ops / (t * 1e-3) / 1e12
+            rec["int8_mfma_frac"] = rec["int8_mfma_tops"] / INT8_MFMA_PEAK_TOPS
+        if form == "auto":
+            rec["chosen_form"] = "int8" if plan.info()["form"] == 1 else "valu"
+            rec["candidates"] = [{"form": "int8" if f == 1 else "valu", "blocks": b,
+                                  "chunk": c, "us": us} for f, b, c, us in plan.tuning()]
+        ntt[form] = rec
+    ntt_equal = all(torch.equal(x, y) for f in ("valu", "auto")
+                    for x, y in zip(nouts[f], nouts["int8"]))
+    ctx.set_plan_forms(ntt="int8")
+    best_b = min(("int8", "fp64", "valu"), key=lambda f: bconv[f]["ms"])
+    best_n = min(("int8", "valu"), key=lambda f: ntt[f]["ms"])
     res = {
         "shape": "64 polynomials x 32 limbs, N=2^16",
-        "bconv_8_to_32_ms": ms_b,
-        "bconv_hbm_gb_s": b_bytes / (ms_b * 1e-3) / 1e9,
-        "bconv_hbm_frac": b_bytes / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "bconv_fp64_mfma_tflop_s": b_flop / (ms_b * 1e-3) / 1e12,
-        "bconv_fp64_mfma_frac": b_flop / (ms_b * 1e-3) / 1e12 / FP64_MFMA_PEAK_TFLOPS,
-        "ntt_fwd_ms": ms_n,
-        "ntt_us_per_limb": 1e3 * ms_n / (npoly * len(dst_p)),
-        "ntt_hbm_gb_s": n_bytes / (ms_n * 1e-3) / 1e9,
-        "ntt_hbm_frac": n_bytes / (ms_n * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "bconv_8_to_32": bconv, "bconv_fastest": best_b, "bconv_forms_equal": bconv_equal,
+        "ntt_forward": ntt, "ntt_fastest": best_n, "ntt_forms_equal": ntt_equal,
+        # the fastest form of each, as the round-1 keys
+        "bconv_8_to_32_ms": bconv[best_b]["ms"], "bconv_hbm_frac": bconv[best_b]["hbm_frac"],
+        "ntt_fwd_ms": ntt[best_n]["ms"], "ntt_us_per_limb": ntt[best_n]["us_per_limb"],
+        "ntt_hbm_frac": ntt[best_n]["hbm_frac"],
     }
-    log(f"config 3: BaseConv 8->32 x 64 polys {ms_b:.3f} ms ({res['bconv_hbm_gb_s']:.0f} GB/s, "
-        f"{res['bconv_fp64_mfma_tflop_s']:.1f} FP64 MFMA TFLOP/s); forward NTT 64 x 32 limbs "
-        f"{ms_n:.3f} ms ({res['ntt_hbm_gb_s']:.0f} GB/s)")
+    log("config 3: BaseConv 8->32 x 64 polys: " +
+        ", ".join(f"{f} {bconv[f]['ms']:.3f} ms ({bconv[f]['hbm_frac']:.2f} HBM)"
+                  for f in bconv) + f"; forms equal: {bconv_equal}")
+    log("config 3: forward NTT 64 x 32 limbs: " +
+        ", ".join(f"{f} {ntt[f]['ms']:.3f} ms ({ntt[f]['hbm_frac']:.2f} HBM)" for f in ntt) +
+        f"; forms equal: {ntt_equal}")
+    if not (bconv_equal and ntt_equal):
+        raise RuntimeError("config-3 forms disagree")
     return res
 
 

@@ -420,10 +420,11 @@ class Context:
         _check(lib.ck_context_set_option(self._h, CK_OPT_PLAN_AUTOTUNE, int(bool(on))))
 
     def set_bconv_mfma(self, on=True):
-        """BaseConv on the FP64 matrix cores (True), fused into the forward
-        NTT column pass (False), or None for the default (matrix cores in
-        batched plans only); outputs are identical."""
-        v = -1 if on is None else int(bool(on))
+        """BaseConv on the FP64 matrix cores (True), on the int8 matrix cores
+        ("int8"), fused into the forward NTT column pass (False), or None for
+        the default (int8 matrix cores in batched plans, fused in single
+        operations); outputs are identical."""
+        v = -1 if on is None else 2 if on == "int8" else int(bool(on))
         _check(lib.ck_context_set_option(self._h, CK_OPT_BCONV_MFMA, v))
 
     def set_plan_forms(self, ntt=None, bconv=None, ntt8_blocks=None, ntt8_chunk=None):
@@ -529,6 +530,7 @@ class Plan:
         _check(lib.ck_plan_info(self._h, out))
         return {"items": out[0], "steps": out[1], "launches": out[2],
                 "workspace_bytes": out[3], "bconv_mfma": bool(out[4]),
+                "bconv_form": ("fused", "fp64", "int8")[out[4]] if out[4] < 3 else None,
                 "tuned_us": (out[5] / 1e3, out[6] / 1e3) if out[5] != -1000 else None,
                 "form": out[7]}
 

@@ -727,28 +727,36 @@ int ck_hrot_hoisted(ck_context *ctx, int level, int n_rot, const int *rots,
                               key_a, moddown != 0, kMaxBsRot, c.ws, sink);
 }
 
-// Builds a plan with the context's BaseConv form, or with autotuning both
-// forms (fused into the forward column pass / on the matrix cores), times
-// each and keeps the faster.
+// Builds a plan with the context's BaseConv form, or with autotuning all
+// three (fused into the forward column pass, on the FP64 matrix cores, on
+// the int8 matrix cores), times each and keeps the fastest.
 extern "C++" template <typename Build>
 static int build_tuned(const Context &c, Build &build, Plan **out) {
   *out = nullptr;
-  if (!c.plan_autotune) return build(c.bconv_mfma != 0, out);
-  Plan *cand[2] = {nullptr, nullptr};
-  double us[2] = {-1.0, -1.0};
-  for (int m = 0; m < 2; m++) {
-    int rc = build(m == 1, &cand[m]);
+  if (!c.plan_autotune) return build(c.bconv_mfma < 0 ? 2 : c.bconv_mfma, out);
+  Plan *best = nullptr;
+  double us[3] = {-1.0, -1.0, -1.0};
+  std::vector<Plan::Candidate> rec;
+  for (int f = 0; f < 3; f++) {
+    Plan *p = nullptr;
+    int rc = build(f, &p);
     if (rc) {
-      plan_destroy(cand[0]);
+      plan_destroy(best);
       return rc;
     }
-    us[m] = plan_time_us(*cand[m], 3);
+    us[f] = plan_time_us(*p, 3);
+    rec.push_back({f, 0, 0, us[f]});
+    if (us[f] >= 0 && (!best || us[f] < us[best->bconv_form])) {
+      plan_destroy(best);
+      best = p;
+    } else {
+      plan_destroy(p);
+    }
   }
-  const int best = (us[1] >= 0 && (us[0] < 0 || us[1] < us[0])) ? 1 : 0;
-  plan_destroy(cand[1 - best]);
-  cand[best]->tuned_us[0] = us[0];
-  cand[best]->tuned_us[1] = us[1];
-  *out = cand[best];
+  if (!best) return fail(CK_ERR_HIP, "autotuning could not time any candidate plan");
+  for (int f = 0; f < 3; f++) best->tuned_us[f] = us[f];
+  best->tuning = rec;
+  *out = best;
   return kOk;
 }
 
@@ -777,11 +785,12 @@ int ck_plan_hmult(ck_context *ctx, int n, const int *levels, int rescale,
     off[i] = total;
     total += relin_ws_limbs(c, levels[i]);
   }
-  auto build = [&](bool mfma, Plan **res) {
+  auto build = [&](int form, Plan **res) {
     Plan *p = new Plan;
     p->ctx = &c;
     p->items = n;
-    p->bconv_mfma = mfma;
+    p->bconv_form = form;
+    p->bconv_mfma = form != 0;
     uint32_t *ws = nullptr;
     int rc = plan_alloc_workspace(*p, total, &ws);
     PlanSink sink(*p);
@@ -881,6 +890,7 @@ int ck_plan_bconv(ck_context *ctx, const int *src_primes, int n_src,
     int rc = kOk;
     if (cd.form == 0) {  // FP64 matrix cores
       p->bconv_mfma = true;
+      p->bconv_form = 1;  // the FP64 matrix-core BaseConv step
       double *d_tab = nullptr;
       rc = plan_upload(*p, &d_tab, ftab);
       PlanSink sink(*p);
@@ -1058,11 +1068,12 @@ int ck_plan_hrot_hoisted(ck_context *ctx, int n, const int *levels,
   for (int r = 0; r < n_rot; r++)
     CK_REQUIRE(rots[r] >= 1 && rots[r] < c.n / 2,
                "hoisted rotation amounts must be in [1, N/2)");
-  auto build = [&](bool mfma, Plan **res) {
+  auto build = [&](int form, Plan **res) {
     Plan *p = new Plan;
     p->ctx = &c;
     p->items = n;
-    p->bconv_mfma = mfma;
+    p->bconv_form = form;
+    p->bconv_mfma = form != 0;
     uint32_t *ws = nullptr;
     int rc = plan_alloc_workspace(*p, total, &ws);
     PlanSink sink(*p);
@@ -1098,7 +1109,7 @@ int ck_plan_info(const ck_plan *plan, long long *out) {
   out[1] = (long long)plan->p->stages.size();
   out[2] = plan_launches(*plan->p);
   out[3] = (long long)plan->p->workspace_bytes;
-  out[4] = plan->p->bconv_mfma ? 1 : 0;
+  out[4] = plan->p->bconv_mfma ? plan->p->bconv_form : 0;
   out[5] = (long long)(plan->p->tuned_us[0] * 1000.0);  // ns, -1000: not tuned
   out[6] = (long long)(plan->p->tuned_us[1] * 1000.0);
   out[7] = plan->p->form;
@@ -1132,7 +1143,7 @@ int ck_context_set_option(ck_context *ctx, int option, int value) {
   CK_REQUIRE(ctx && ctx->c, "null context");
   switch (option) {
     case CK_OPT_BCONV_MFMA:
-      CK_REQUIRE(value >= -1 && value <= 1, "option value must be -1, 0 or 1");
+      CK_REQUIRE(value >= -1 && value <= 2, "option value must be -1, 0, 1 or 2");
       ctx->c->bconv_mfma = value;
       return CK_OK;
     case CK_OPT_PLAN_AUTOTUNE:

@@ -467,6 +467,7 @@ void prof_free(const Context &c);
 void context_destroy(Context *ctx) {
   if (!ctx) return;
   prof_free(*ctx);
+  for (auto &e : ctx->i8_bconv_planes) (void)hipFree(e.second);
   for (void *p : ctx->allocations) (void)hipFree(p);
   delete ctx;
 }

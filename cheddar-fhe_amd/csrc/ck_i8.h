@@ -2,6 +2,7 @@
 // ck_i8.hip), and a vector-unit BaseConv used as their baseline.
 #pragma once
 
+#include "ck_bconv.h"
 #include "ck_i8_arith.h"
 #include "ck_internal.h"
 
@@ -23,7 +24,7 @@ constexpr int kI8Ctx = 35;
 struct I8NttTabs {
   const int32_t *planes = nullptr;  // [prime][kI8Ctx][4 planes][64 lanes][4]
   const int32_t *corr = nullptr;    // [prime][2][16] offset corrections
-  const uint32_t *tw = nullptr;     // [prime][256][256] psi^(c (2 brv8(p) + 1))
+  const uint32_t *tw = nullptr;     // [prime] psi^(c (2 brv8(p) + 1)), 16 x 16 tiles
 };
 
 // One limb of a batched int8 transform: dst = NTT(src) (x R when mont).
@@ -64,8 +65,26 @@ struct BconvI8Inst {
   uint8_t dst_prime[kI8MaxDst];
 };
 
+constexpr int kMaxBconvI8Inst = 4;  // instances per kernel-argument launch
+struct BconvI8Launch {
+  int n;
+  BconvI8Inst inst[kMaxBconvI8Inst];
+};
+
 void launch_bconv_i8_table(const Context &c, const BconvI8Inst *insts, int n, int dsts,
                            int max_s, int max_d, hipStream_t st);
+// kernel-argument form (single operations: no device table)
+void launch_bconv_i8(const Context &c, const BconvI8Launch &L, hipStream_t st);
+
+// The int8 operand planes of a context BaseConv table (a ModUp group's or a
+// ModDown plan's FP64 {hi, lo} table on the device, S sources x D
+// destinations), built on first use from the table's words and kept in the
+// context. Returns nullptr (with the error set) on failure.
+const int64_t *bconv_i8_planes_for(const Context &c, const double *d_ftab, int S, int D,
+                                   const uint8_t *dst_prime);
+// The int8 form of a FP64-table BaseConv instance (same source, outputs
+// and destinations); false on failure.
+bool to_i8_inst(const Context &c, const BconvInst &in, BconvI8Inst *out);
 void launch_bconv_valu_table(const Context &c, const BconvI8Inst *insts, int n, int dsts,
                              int max_s, hipStream_t st);
 

@@ -78,11 +78,11 @@ __device__ __forceinline__ v4i mfma64(v4i a, v4i b, v4i c) {
 typedef const __attribute__((address_space(4))) Ntt8Job CJob;
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 
-// data streams through once: non-temporal accesses keep it from evicting the
-// twiddle table and operand matrices, which every limb of a prime re-reads
-__device__ __forceinline__ uint32_t ld_nt(const uint32_t *p) {
-  return __builtin_nontemporal_load(p);
-}
+// data that streams through once (the intermediate tiles, the outputs):
+// non-temporal accesses keep it from evicting the twiddle table and operand
+// matrices, which every limb of a prime re-reads. The column pass reads its
+// input with ordinary loads: it uses each 128-byte line in two 64-byte
+// halves, and a line kept in L2 serves the second half.
 __device__ __forceinline__ v4u ld4_nt(const uint32_t *p) {
   return __builtin_nontemporal_load((const v4u *)p);
 }
@@ -103,13 +103,52 @@ __device__ __forceinline__ int tile_word(int p1, int cb, int g, int n) {
   return (16 * p1 + cb) * 256 + (16 * g + n) * 4;
 }
 
-// Column pass: workgroup = one limb's 16-column blocks [part * cbs, +cbs).
-// Wave w runs the first stage for r0 = 4w..4w+3 and the second stage for
-// p1 = 4w..4w+3, so it keeps A1 and four B1 matrices in registers. The
-// next block's inputs are loaded while the current one is transformed.
+// Four planes of one 16 x 16 output tile (matrix a[j] times packed data b,
+// plane 0 starting from c0).
+struct Acc4 {
+  v4i d[4];
+};
+__device__ __forceinline__ Acc4 mfma_tile(const v4i *a, v4i b, v4i c0) {
+  const v4i zero = {0, 0, 0, 0};
+  Acc4 r;
+  r.d[0] = mfma64(a[0], b, c0);
+  r.d[1] = mfma64(a[1], b, zero);
+  r.d[2] = mfma64(a[2], b, zero);
+  r.d[3] = mfma64(a[3], b, zero);
+  return r;
+}
+// data as the A operand (rows = data vectors), matrix as B
+__device__ __forceinline__ Acc4 mfma_tile_t(v4i a, const v4i *b) {
+  const v4i zero = {0, 0, 0, 0};
+  Acc4 r;
+  r.d[0] = mfma64(a, b[0], zero);
+  r.d[1] = mfma64(a, b[1], zero);
+  r.d[2] = mfma64(a, b[2], zero);
+  r.d[3] = mfma64(a, b[3], zero);
+  return r;
+}
+// the four outputs of a lane, recombined and reduced, each step over all
+// four before the next (four independent dependency chains)
+__device__ __forceinline__ void reduce4(const Acc4 &A, double q, double qr, double r[4]) {
+  double v[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) v[i] = combine(A.d[0][i], A.d[1][i], A.d[2][i], A.d[3][i]);
+#pragma unroll
+  for (int i = 0; i < 4; i++) r[i] = reduce_q(v[i], q, qr);
+}
+
+// Column pass: workgroup part of a limb's `parts` = 16 / cbs workgroups
+// takes the 16-column blocks part, part + parts, ...: with two parts the
+// workgroups of a limb (adjacent, so on one XCD and started together) read
+// the two 64-byte halves of each 128-byte input line at about the same
+// time. Wave w runs the first stage for r0 = 4w..4w+3 and the second stage for
+// p1 = 4w..4w+3, so it keeps A1 and four B1 matrices in registers. Each
+// stage is software-pipelined over its four tiles: the next tile's matrix
+// products (and LDS reads) are issued before the current tile's
+// recombination, and the next block's inputs load during the block.
 __global__ __launch_bounds__(256, 3) void ntt8_cols_kernel(const Ntt8Job *__restrict__ jobs,
-                                                        int nwg, int cbs, I8NttTabs T,
-                                                        PrimeTab P) {
+                                                           int nwg, int cbs, I8NttTabs T,
+                                                           PrimeTab P) {
   __shared__ __align__(16) int xs[2][16 * kXs];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int g = lane >> 4, n = lane & 15;
@@ -134,72 +173,76 @@ __global__ __launch_bounds__(256, 3) void ntt8_cols_kernel(const Ntt8Job *__rest
   const v4i corr = {cp[0], cp[1], cp[2], cp[3]};
   const v4i zero = {0, 0, 0, 0};
   const double q = P.qd[pr], qr = P.qr[pr];
-  const uint32_t *__restrict__ tw = T.tw + (size_t)pr * kN;
-
+  // the twiddle table is tiled like the intermediate: one 16-byte load
+  // gives a lane the four twiddles of its outputs in tile (p1, cb)
+  const uint32_t *__restrict__ tw0 = T.tw + (size_t)pr * kN + (16 * g + n) * 4;
   // lane's first input word of a block: row 4w + 64 g, column n
   const uint32_t *in0 = src + (4 * w + 64 * g) * 256 + n;
-  const int cb0 = part * cbs, cb1 = cb0 + cbs;
+  // this lane's 16-byte slot of tile (p1, cb)
+  uint32_t *out0 = dst + (16 * g + n) * 4;
+
   uint32_t raw[4][4];
 #pragma unroll
   for (int k = 0; k < 4; k++)
 #pragma unroll
-    for (int t = 0; t < 4; t++) raw[k][t] = ld_nt(in0 + (k + 16 * t) * 256 + cb0 * 16);
+    for (int t = 0; t < 4; t++) raw[k][t] = in0[(k + 16 * t) * 256 + part * 16];
 
-  for (int cb = cb0; cb < cb1; cb++) {
-    int *X = xs[cb & 1];
-    const int col = cb * 16 + n;
-    // next block's inputs (the last iteration reloads its own block)
-    const int cbn = cb + 1 < cb1 ? cb + 1 : cb;
-    uint32_t nxt[4][4];
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-#pragma unroll
-      for (int t = 0; t < 4; t++) nxt[k][t] = ld_nt(in0 + (k + 16 * t) * 256 + cbn * 16);
+  for (int it = 0; it < cbs; it++) {
+    const int cb = part + parts * it;
+    int *X = xs[it & 1];
     // first stage: 16-point transforms along r1, tile r0 = 4w + k
+    Acc4 acc[2];
+    {
+      const v4i b = {pack_off(raw[0][0]), pack_off(raw[0][1]), pack_off(raw[0][2]),
+                     pack_off(raw[0][3])};
+      acc[0] = mfma_tile(a1, b, corr);
+    }
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const v4i b = {pack_off(raw[k][0]), pack_off(raw[k][1]), pack_off(raw[k][2]),
-                     pack_off(raw[k][3])};
-      const v4i d0 = mfma64(a1[0], b, corr);
-      const v4i d1 = mfma64(a1[1], b, zero);
-      const v4i d2 = mfma64(a1[2], b, zero);
-      const v4i d3 = mfma64(a1[3], b, zero);
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const double r = reduce_q(combine(d0[v], d1[v], d2[v], d3[v]), q, qr);
-        X[(4 * g + v) * kXs + (4 * w + k) * 20 + n] = pack_c((int)r);
+      if (k < 3) {
+        const v4i b = {pack_off(raw[k + 1][0]), pack_off(raw[k + 1][1]),
+                       pack_off(raw[k + 1][2]), pack_off(raw[k + 1][3])};
+        acc[(k + 1) & 1] = mfma_tile(a1, b, corr);
       }
+      double r[4];
+      reduce4(acc[k & 1], q, qr, r);
+#pragma unroll
+      for (int v = 0; v < 4; v++) X[(4 * g + v) * kXs + (4 * w + k) * 20 + n] = pack_c((int)r[v]);
+    }
+    // next block's inputs, in flight during the second stage (the last
+    // iteration reloads its own block)
+    {
+      const int cbn = it + 1 < cbs ? cb + parts : cb;
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) raw[k][t] = in0[(k + 16 * t) * 256 + cbn * 16];
     }
     __syncthreads();
-    uint32_t twv[4][4];
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) twv[k][v] = tw[(4 * g + v + 16 * (4 * w + k)) * 256 + col];
     // second stage along r0 (tile p1 = 4w + k), then the 4-step twiddle;
     // one 16-byte store per lane writes the tile (p1, cb)
+    v4u twv[2];
+    twv[0] = *(const v4u *)(tw0 + (16 * (4 * w) + cb) * 256);
+    v4i bx[2];
+    const int *Xl = X + (4 * w) * kXs + (4 * g) * 20 + n;
+    bx[0] = v4i{Xl[0], Xl[20], Xl[40], Xl[60]};
+    acc[0] = mfma_tile(b1[0], bx[0], zero);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const int p1 = 4 * w + k;
-      const v4i b = {X[p1 * kXs + (4 * g) * 20 + n], X[p1 * kXs + (4 * g + 1) * 20 + n],
-                     X[p1 * kXs + (4 * g + 2) * 20 + n], X[p1 * kXs + (4 * g + 3) * 20 + n]};
-      const v4i d0 = mfma64(b1[k][0], b, zero);
-      const v4i d1 = mfma64(b1[k][1], b, zero);
-      const v4i d2 = mfma64(b1[k][2], b, zero);
-      const v4i d3 = mfma64(b1[k][3], b, zero);
+      if (k < 3) {
+        const int *Xk = Xl + (k + 1) * kXs;
+        bx[(k + 1) & 1] = v4i{Xk[0], Xk[20], Xk[40], Xk[60]};
+        twv[(k + 1) & 1] = *(const v4u *)(tw0 + (16 * (4 * w + k + 1) + cb) * 256);
+        acc[(k + 1) & 1] = mfma_tile(b1[k + 1], bx[(k + 1) & 1], zero);
+      }
+      double r[4];
+      reduce4(acc[k & 1], q, qr, r);
       v4u out;
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const double r = reduce_q(combine(d0[v], d1[v], d2[v], d3[v]), q, qr);
-        const double z = mulmod_fq(r, (double)twv[k][v], q, qr);
-        out[v] = (uint32_t)(int)z;
-      }
-      st4_nt(dst + tile_word(p1, cb, g, n), out);
+      for (int v = 0; v < 4; v++)
+        out[v] = (uint32_t)(int)mulmod_fq(r[v], (double)twv[k & 1][v], q, qr);
+      st4_nt(out0 + (16 * (4 * w + k) + cb) * 256, out);
     }
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-#pragma unroll
-      for (int t = 0; t < 4; t++) raw[k][t] = nxt[k][t];
   }
 }
 
@@ -207,11 +250,11 @@ __global__ __launch_bounds__(256, 3) void ntt8_cols_kernel(const Ntt8Job *__rest
 // [part * rbs, +rbs). Wave w runs the first stage for rows 16 rb + 4w + k
 // (one 16-byte load per lane and tile gives its four rows) and the second
 // stage for q1 = 4w + k with the data as the A operand, so each lane ends
-// with four rows of one output column. The next block's tiles are loaded
-// while the current one is transformed.
-__global__ __launch_bounds__(256) void ntt8_rows_kernel(const Ntt8Job *__restrict__ jobs,
-                                                        int nwg, int rbs, I8NttTabs T,
-                                                        PrimeTab P) {
+// with four rows of one output column. Software-pipelined like the column
+// pass; the next block's tiles load while the current one is transformed.
+__global__ __launch_bounds__(256, 3) void ntt8_rows_kernel(const Ntt8Job *__restrict__ jobs,
+                                                           int nwg, int rbs, I8NttTabs T,
+                                                           PrimeTab P) {
   __shared__ __align__(16) int xs[2][16 * kXs];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int g = lane >> 4, n = lane & 15;
@@ -235,51 +278,57 @@ __global__ __launch_bounds__(256) void ntt8_rows_kernel(const Ntt8Job *__restric
 
   const int rb0 = part * rbs, rb1 = rb0 + rbs;
   // tile (rb, c1 = 4g + t), rows 4w + k (k = 0..3 in one load), column c0 = n
+  const uint32_t *in0 = Z + tile_word(0, 4 * g, w, n);
   v4u raw[4];
 #pragma unroll
-  for (int t = 0; t < 4; t++) raw[t] = ld4_nt(Z + tile_word(rb0, 4 * g + t, w, n));
+  for (int t = 0; t < 4; t++) raw[t] = ld4_nt(in0 + (16 * rb0 + t) * 256);
 
   for (int rb = rb0; rb < rb1; rb++) {
     int *W = xs[rb & 1];
-    const int rbn = rb + 1 < rb1 ? rb + 1 : rb;
-    v4u nxt[4];
-#pragma unroll
-    for (int t = 0; t < 4; t++) nxt[t] = ld4_nt(Z + tile_word(rbn, 4 * g + t, w, n));
     // first stage along c1 (row 16 rb + 4w + k): V[q1 = 4g + v][c0 = n]
+    Acc4 acc[2];
+    {
+      const v4i b = {pack_c((int)raw[0][0]), pack_c((int)raw[1][0]), pack_c((int)raw[2][0]),
+                     pack_c((int)raw[3][0])};
+      acc[0] = mfma_tile(a3, b, zero);
+    }
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const v4i b = {pack_c((int)raw[0][k]), pack_c((int)raw[1][k]), pack_c((int)raw[2][k]),
-                     pack_c((int)raw[3][k])};
-      const v4i d0 = mfma64(a3[0], b, zero);
-      const v4i d1 = mfma64(a3[1], b, zero);
-      const v4i d2 = mfma64(a3[2], b, zero);
-      const v4i d3 = mfma64(a3[3], b, zero);
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const double r = reduce_q(combine(d0[v], d1[v], d2[v], d3[v]), q, qr);
-        W[(4 * w + k) * kXs + (4 * g + v) * 20 + n] = pack_c((int)r);
+      if (k < 3) {
+        const v4i b = {pack_c((int)raw[0][k + 1]), pack_c((int)raw[1][k + 1]),
+                       pack_c((int)raw[2][k + 1]), pack_c((int)raw[3][k + 1])};
+        acc[(k + 1) & 1] = mfma_tile(a3, b, zero);
       }
+      double r[4];
+      reduce4(acc[k & 1], q, qr, r);
+#pragma unroll
+      for (int v = 0; v < 4; v++) W[(4 * w + k) * kXs + (4 * g + v) * 20 + n] = pack_c((int)r[v]);
+    }
+    // next block's tiles, in flight during the second stage
+    {
+      const int rbn = rb + 1 < rb1 ? rb + 1 : rb;
+#pragma unroll
+      for (int t = 0; t < 4; t++) raw[t] = ld4_nt(in0 + (16 * rbn + t) * 256);
     }
     __syncthreads();
     // the row block's tiles are all read: the outputs may overwrite them
     // second stage along c0 (q1 = 4w + k): rows as A (row n, c0 = 4g + t)
+    const v4i *Wl = (const v4i *)&W[n * kXs + (4 * w) * 20 + 4 * g];
+    uint32_t *out0 = Z + (16 * rb + 4 * g) * 256 + n + 16 * (4 * w);
+    v4i ax[2];
+    ax[0] = Wl[0];
+    acc[0] = mfma_tile_t(ax[0], b3[0]);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const int q1 = 4 * w + k;
-      const v4i a = *(const v4i *)&W[n * kXs + q1 * 20 + 4 * g];
-      const v4i d0 = mfma64(a, b3[k][0], zero);
-      const v4i d1 = mfma64(a, b3[k][1], zero);
-      const v4i d2 = mfma64(a, b3[k][2], zero);
-      const v4i d3 = mfma64(a, b3[k][3], zero);
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int row = 16 * rb + 4 * g + v;
-        const double r = reduce_q(combine(d0[v], d1[v], d2[v], d3[v]), q, qr);
-        st_nt(Z + row * 256 + n + 16 * q1, canon_small(r, q));
+      if (k < 3) {
+        ax[(k + 1) & 1] = Wl[5 * (k + 1)];  // + (k + 1) * 20 words
+        acc[(k + 1) & 1] = mfma_tile_t(ax[(k + 1) & 1], b3[k + 1]);
       }
-    }
+      double r[4];
+      reduce4(acc[k & 1], q, qr, r);
 #pragma unroll
-    for (int t = 0; t < 4; t++) raw[t] = nxt[t];
+      for (int v = 0; v < 4; v++) st_nt(out0 + v * 256 + 16 * k, canon_small(r[v], q));
+    }
   }
 }
 
@@ -292,10 +341,8 @@ typedef const __attribute__((address_space(4))) BconvI8Inst CBInst;
 // digits per v_mfma_i32_16x16x32_i8 (KS k-steps for up to 8 KS sources).
 // Source indices past S are clamped to 0 (their plane bytes are 0), so every
 // load is unconditional.
-template <int KS>
-__global__ __launch_bounds__(256) void bconv_i8_kernel(const BconvI8Inst *__restrict__ insts,
-                                                       PrimeTab P) {
-  CBInst &I = ((CBInst *)insts)[blockIdx.y];
+template <int KS, class Inst>
+__device__ __forceinline__ void bconv_i8_body(Inst &I, const PrimeTab &P) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int g = lane >> 4, n = lane & 15;
   const int S = I.S, D = I.D, canonical = I.canonical;
@@ -362,6 +409,19 @@ __global__ __launch_bounds__(256) void bconv_i8_kernel(const BconvI8Inst *__rest
       }
     }
   }
+}
+
+template <int KS>
+__global__ __launch_bounds__(256) void bconv_i8_kernel(const BconvI8Inst *__restrict__ insts,
+                                                       PrimeTab P) {
+  CBInst &I = ((CBInst *)insts)[blockIdx.y];
+  bconv_i8_body<KS>(I, P);
+}
+
+template <int KS>
+__global__ __launch_bounds__(256) void bconv_i8_arg_kernel(const BconvI8Launch L,
+                                                           PrimeTab P) {
+  bconv_i8_body<KS>(L.inst[blockIdx.y], P);
 }
 
 // Vector-unit form: one position per thread, the table row of each
@@ -457,13 +517,16 @@ int ntt8_tables(Context &c, I8NttTabs *out) {
           T[q0][c0] = h_powmod(om, (uint64_t)c0 * brv_n(q1, 4) + 16ull * c0 * brv_n(q0, 4), q);
       i8::ctx_planes(T, q, pp + (size_t)(kI8CtxB3 + q1) * 1024);
     }
-    // 4-step twiddle psi^(c (2 brv8(p) + 1)), row p, column c
+    // 4-step twiddle psi^(c (2 brv8(p) + 1)) of row p, column c, stored in
+    // the tiled layout of the intermediate (tile_word)
     uint32_t *tp = tw.data() + (size_t)pi * kN;
     for (int p = 0; p < 256; p++) {
       const uint32_t step = h_powmod(psi, 2 * brv_n(p, 8) + 1, q);
       uint64_t x = 1;
+      const int p1 = p >> 4, p0 = p & 15;
       for (int col = 0; col < 256; col++) {
-        tp[p * 256 + col] = (uint32_t)x;
+        const int cb = col >> 4, cn = col & 15;
+        tp[(16 * p1 + cb) * 256 + (16 * (p0 >> 2) + cn) * 4 + (p0 & 3)] = (uint32_t)x;
         x = x * step % q;
       }
     }
@@ -514,6 +577,65 @@ void launch_bconv_i8_table(const Context &c, const BconvI8Inst *insts, int n, in
   else
     launch(c, "bconv_i8", dsts, bconv_i8_kernel<2>, dim3(kN / 256, n), dim3(256), st, insts,
            c.ptab);
+}
+
+void launch_bconv_i8(const Context &c, const BconvI8Launch &L, hipStream_t st) {
+  if (L.n <= 0) return;
+  int dsts = 0, max_s = 1;
+  for (int i = 0; i < L.n; i++) {
+    dsts += L.inst[i].D;
+    max_s = std::max(max_s, L.inst[i].S);
+  }
+  if (max_s <= 8)
+    launch(c, "bconv_i8", dsts, bconv_i8_arg_kernel<1>, dim3(kN / 256, L.n), dim3(256), st,
+           L, c.ptab);
+  else
+    launch(c, "bconv_i8", dsts, bconv_i8_arg_kernel<2>, dim3(kN / 256, L.n), dim3(256), st,
+           L, c.ptab);
+}
+
+const int64_t *bconv_i8_planes_for(const Context &c, const double *d_ftab, int S, int D,
+                                   const uint8_t *dst_prime) {
+  for (auto &e : c.i8_bconv_planes)
+    if (e.first == d_ftab) return e.second;
+  std::vector<double> f((size_t)2 * S * D);
+  if (check_hip(hipMemcpy(f.data(), d_ftab, f.size() * sizeof(double), hipMemcpyDeviceToHost),
+                "hipMemcpy(BaseConv table)"))
+    return nullptr;
+  std::vector<uint32_t> tab((size_t)S * D), dq(D);
+  for (int d = 0; d < D; d++) {
+    dq[d] = c.primes[dst_prime[d]];
+    for (int i = 0; i < S; i++)
+      tab[(size_t)d * S + i] = (uint32_t)f[2 * ((size_t)d * S + i)] * 65536u +
+                               (uint32_t)f[2 * ((size_t)d * S + i) + 1];
+  }
+  const std::vector<int64_t> pl = i8::bconv_i8_planes(tab, S, D, dq);
+  int64_t *dp = nullptr;
+  if (check_hip(hipMalloc((void **)&dp, pl.size() * sizeof(int64_t)), "hipMalloc(int8 planes)"))
+    return nullptr;
+  if (check_hip(hipMemcpy(dp, pl.data(), pl.size() * sizeof(int64_t), hipMemcpyHostToDevice),
+                "hipMemcpy(int8 planes)")) {
+    (void)hipFree(dp);
+    return nullptr;
+  }
+  c.i8_bconv_planes.emplace_back(d_ftab, dp);
+  return dp;
+}
+
+bool to_i8_inst(const Context &c, const BconvInst &in, BconvI8Inst *out) {
+  std::memset(out, 0, sizeof(*out));
+  out->planes = bconv_i8_planes_for(c, in.tab, in.S, in.D, in.dst_prime);
+  if (!out->planes) return false;
+  out->src = in.src;
+  out->dst = in.dst;
+  out->S = in.S;
+  out->D = in.D;
+  out->canonical = in.canonical;
+  for (int m = 0; m < in.D; m++) {
+    out->dst_limb[m] = in.dst_limb[m];
+    out->dst_prime[m] = in.dst_prime[m];
+  }
+  return true;
 }
 
 void launch_bconv_valu_table(const Context &c, const BconvI8Inst *insts, int n, int dsts,

@@ -158,10 +158,14 @@ struct Context {
 
   std::vector<LevelPlan> levels;
 
-  // BaseConv on the FP64 matrix cores (bconv_mfma) instead of fused into
-  // the forward column pass: 1 always, 0 never, -1 (default) for batched
-  // plans only, where it measured faster
+  // BaseConv as its own step on the matrix cores instead of fused into the
+  // forward column pass: 1 FP64 matrix cores everywhere, 2 int8 matrix
+  // cores everywhere, 0 fused everywhere, -1 (default) int8 matrix cores in
+  // batched plans and fused in single operations (measured fastest)
   int bconv_mfma = -1;
+  // int8 operand planes of the BaseConv tables, by FP64 table (built on
+  // first use, freed with the context)
+  mutable std::vector<std::pair<const double *, int64_t *>> i8_bconv_planes;
   // batched plans: build both BaseConv forms, time them, keep the faster
   bool plan_autotune = false;
   // key switching: the ModUp's forward row stages fused with the key inner
@@ -170,8 +174,8 @@ struct Context {
   bool fuse_key_rows = true;
   // forms of ck_plan_ntt (forward) and ck_plan_bconv: see CK_OPT_PLAN_NTT_FORM
   // and CK_OPT_PLAN_BCONV_FORM in cheddar_amd.h (-1: autotune)
-  int plan_ntt_form = 0;
-  int plan_bconv_form = 0;
+  int plan_ntt_form = 1;    // int8 matrix cores (measured faster)
+  int plan_bconv_form = 1;  // int8 matrix cores (measured faster)
   int ntt8_blocks = 16;  // int8 transform: 16-column blocks per workgroup
   int ntt8_chunk = 0;    // int8 transform: limbs per column/row chunk (0: all)
 

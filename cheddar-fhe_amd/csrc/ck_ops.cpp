@@ -17,6 +17,7 @@
 //     instead of being copied,
 //   * all polynomials of a ModDown share its launches.
 #include "ck_ops.h"
+#include "ck_i8.h"
 
 #include <algorithm>
 #include <cstring>
@@ -115,6 +116,17 @@ int ImmediateSink::copy(uint32_t *dst, const uint32_t *src, size_t words) {
 }
 
 int ImmediateSink::bconv(const std::vector<BconvInst> &insts) {
+  if (c_.bconv_mfma == 2) {  // int8 matrix cores
+    for (size_t i = 0; i < insts.size(); i += kMaxBconvI8Inst) {
+      BconvI8Launch L;
+      std::memset(&L, 0, sizeof(L));
+      L.n = (int)std::min<size_t>(kMaxBconvI8Inst, insts.size() - i);
+      for (int k = 0; k < L.n; k++)
+        if (!to_i8_inst(c_, insts[i + k], &L.inst[k])) return kErrHip;
+      launch_bconv_i8(c_, L, st_);
+    }
+    return launch_status("bconv");
+  }
   for (size_t i = 0; i < insts.size(); i += kMaxBconvInst) {
     BconvLaunch L;
     std::memset(&L, 0, sizeof(L));

@@ -55,7 +55,7 @@ class ImmediateSink : public Sink {
   int pass(bool forward, int flags, const std::vector<PassJob> &jobs) override;
   int copy(uint32_t *dst, const uint32_t *src, size_t words) override;
   int bconv(const std::vector<BconvInst> &insts) override;
-  bool mfma_bconv() const override { return c_.bconv_mfma == 1; }
+  bool mfma_bconv() const override { return c_.bconv_mfma == 1 || c_.bconv_mfma == 2; }
   int key_accum(const EwNp &np, const KeyAccumArgs &A) override;
   int key_rows(const EwNp &np, const KeyAccumArgs &A) override;
   int bs_fused(const EwNp &np, const BsFusedArgs &A) override;

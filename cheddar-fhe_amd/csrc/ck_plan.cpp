@@ -75,6 +75,15 @@ int PlanSink::copy(uint32_t *, const uint32_t *, size_t) {
 }
 
 int PlanSink::bconv(const std::vector<BconvInst> &insts) {
+  if (plan_.bconv_form == 2) {  // int8 matrix cores
+    PlanStage &s = stage(PlanStage::kBconvI8);
+    for (const BconvInst &I : insts) {
+      BconvI8Inst J;
+      if (!to_i8_inst(*plan_.ctx, I, &J)) return kErrHip;
+      s.bi.push_back(J);
+    }
+    return kOk;
+  }
   PlanStage &s = stage(PlanStage::kBconv);
   s.bc.insert(s.bc.end(), insts.begin(), insts.end());
   return kOk;

@@ -40,9 +40,10 @@ struct Plan {
   size_t workspace_bytes = 0;
   int items = 0;
   bool bconv_mfma = true;  // BaseConv steps on the matrix cores
+  int bconv_form = 2;       // 0 fused, 1 FP64 matrix cores, 2 int8 matrix cores
   // autotuning record: measured run time of each BaseConv form (us; < 0:
-  // not measured)
-  double tuned_us[2] = {-1.0, -1.0};
+  // not measured): fused, FP64 matrix cores, int8 matrix cores
+  double tuned_us[3] = {-1.0, -1.0, -1.0};
   // the form of a transform or BaseConv plan (ck_plan_info) and every
   // candidate that autotuning timed: form, blocks, chunk, run time (us)
   int form = 0;

@@ -340,7 +340,8 @@ int ck_plan_ntt(ck_context *ctx, int n, int num_main, int num_ter, int num_aux,
                 uint32_t *const *dst, ck_plan **out);
 int ck_plan_run(ck_plan *plan, void *stream);
 /* out[8]: items, steps, kernel launches per run, workspace bytes, BaseConv
- * form of an HMult / hoisted plan (1: matrix cores, 0: fused), with
+ * form of an HMult / hoisted plan (0: fused, 1: FP64 matrix cores, 2: int8
+ * matrix cores), with
  * CK_OPT_PLAN_AUTOTUNE the measured run time of each of those forms in ns
  * (fused, matrix cores; -1000 when not tuned), and the form of a transform
  * or BaseConv plan (CK_OPT_PLAN_NTT_FORM / CK_OPT_PLAN_BCONV_FORM values) */
@@ -357,17 +358,19 @@ int ck_plan_destroy(ck_plan *plan);
 
 /* ---- Options ------------------------------------------------------------------ */
 /* CK_OPT_BCONV_MFMA: 1 runs every BaseConv (ModUp, ModDown, Rescale) on the
- * FP64 matrix cores as its own step, 0 fuses it into the forward NTT's
- * column pass, -1 (default) uses the matrix cores for batched plans only;
- * results are identical. No reference counterpart (the
+ * FP64 matrix cores as its own step, 2 on the int8 matrix cores (residues
+ * and table split into bytes), 0 fuses it into the forward NTT's column
+ * pass, -1 (default) uses the int8 matrix cores for batched plans and the
+ * fused form for single operations; results are identical. No reference counterpart (the
  * reference's kFuseModDownEpilogue-style compile-time switches,
  * include/core/Parameter.h:13-20, select kernel fusions the same way). */
 #define CK_OPT_BCONV_MFMA 1
 /* CK_OPT_PLAN_AUTOTUNE: 1 makes ck_plan_hmult / ck_plan_hrot_hoisted build
- * the plan with both BaseConv forms, run each (one warm-up + 3 timed runs
- * on a private stream, synchronising the host: plan creation only; the runs
- * write the plan's outputs) and keep the faster; 0 (default) uses
- * CK_OPT_BCONV_MFMA. */
+ * the plan with each BaseConv form (fused, FP64 matrix cores, int8 matrix
+ * cores), run each (one warm-up + 3 timed runs on a private stream,
+ * synchronising the host: plan creation only; the runs write the plan's
+ * outputs) and keep the fastest (ck_plan_tuning lists the times); 0
+ * (default) uses CK_OPT_BCONV_MFMA. */
 #define CK_OPT_PLAN_AUTOTUNE 2
 /* CK_OPT_FUSED_KEY_ROWS: 1 (default) runs the last ModUp transform stages
  * (forward row stages) and the key inner product of every key switch
@@ -376,19 +379,19 @@ int ck_plan_destroy(ck_plan *plan);
  * identical. */
 #define CK_OPT_FUSED_KEY_ROWS 3
 /* CK_OPT_PLAN_NTT_FORM: the form of forward transforms in ck_plan_ntt.
- * 0 (default): radix-16 FP64 butterflies on the vector unit (NTTPhase1 /
- * NTTPhase2, src/core/NTT.cu:231-454); 1: the 4-step 256 x 256 transform as
- * exact int8 matrix products (residues split into four bytes, int32
- * accumulation, exact recombination); -1: autotune, i.e. build the vector
+ * 0: radix-16 FP64 butterflies on the vector unit (NTTPhase1 / NTTPhase2,
+ * src/core/NTT.cu:231-454); 1 (default, measured faster): the 4-step
+ * 256 x 256 transform as exact int8 matrix products (residues split into
+ * four bytes, int32 accumulation, exact recombination); -1: autotune, i.e. build the vector
  * form and the int8 form for every (CK_OPT_NTT8_BLOCKS, CK_OPT_NTT8_CHUNK)
  * candidate, time each and keep the fastest (ck_plan_tuning lists them;
  * outputs must not alias inputs). Inverse transforms always use the vector
  * form. Results are identical. */
 #define CK_OPT_PLAN_NTT_FORM 4
-/* CK_OPT_PLAN_BCONV_FORM: the form of ck_plan_bconv. 0 (default): FP64
- * matrix cores (16-bit table halves); 1: int8 matrix cores (residues and
- * table split into bytes); 2: vector unit (exact FP64 FMAs); -1: autotune
- * over the three. Results are identical. */
+/* CK_OPT_PLAN_BCONV_FORM: the form of ck_plan_bconv. 0: FP64 matrix cores
+ * (16-bit table halves); 1 (default, measured faster): int8 matrix cores
+ * (residues and table split into bytes); 2: vector unit (exact FP64 FMAs);
+ * -1: autotune over the three. Results are identical. */
 #define CK_OPT_PLAN_BCONV_FORM 5
 /* CK_OPT_NTT8_BLOCKS: 16-column (16-row) blocks of one limb per workgroup
  * of the int8 transform's column (row) pass: 1, 2, 4, 8 or 16 (default). */

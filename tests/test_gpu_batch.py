@@ -22,13 +22,15 @@ def ck():
 
 
 @pytest.fixture(scope="module",
-                params=["bconv-fused", "bconv-mfma", "autotune", "key-accum-pass"])
+                params=["bconv-fused", "bconv-mfma", "bconv-int8", "autotune",
+                        "key-accum-pass"])
 def ctx24(ck, rns24, request):
     ctx = ck.Context.from_json(rns24)
     if request.param == "autotune":
         ctx.set_plan_autotune(True)
     else:
-        ctx.set_bconv_mfma(request.param == "bconv-mfma")
+        ctx.set_bconv_mfma("int8" if request.param == "bconv-int8"
+                           else request.param == "bconv-mfma")
     # key-accum-pass: the key inner product as its own step, not fused with
     # the ModUp's forward row stages
     ctx.fused_key_rows = request.param != "key-accum-pass"
@@ -64,10 +66,11 @@ def test_plan_hmult_mixed_levels(ck, ctx24, oracle_rns24, key24):
     assert info["items"] == len(levels)
     assert info["launches"] == ((11 if info["bconv_mfma"] else 9)
                                 - (1 if ctx24.fused_key_rows else 0))
-    if info["tuned_us"] is not None:  # autotuned: the kept form is the faster
-        fused, mfma = info["tuned_us"]
-        assert min(fused, mfma) > 0
-        assert info["bconv_mfma"] == (mfma < fused)
+    tuning = plan.tuning()
+    if tuning:  # autotuned: all three BaseConv forms timed, the fastest kept
+        assert sorted(t[0] for t in tuning) == [0, 1, 2] and min(t[3] for t in tuning) > 0
+        assert ("fused", "fp64", "int8")[min(tuning, key=lambda t: t[3])[0]] == \
+            info["bconv_form"]
     for rep in range(2):
         for o in outs:
             o[0].fill_(-1)

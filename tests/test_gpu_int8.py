@@ -78,7 +78,7 @@ def test_ntt8_plan_rns24(ck, ctx24, oracle_rns24, blocks, chunk, mont):
         nchunk = 1 if chunk == 0 else -(-5 * 32 // chunk)
         assert info["form"] == 1 and info["launches"] == 2 * nchunk
     finally:
-        ctx24.set_plan_forms(ntt="valu", ntt8_blocks=16, ntt8_chunk=0)
+        ctx24.set_plan_forms(ntt="int8", ntt8_blocks=16, ntt8_chunk=0)
 
 
 @pytest.mark.parametrize("np3", [(12, 0, 3), (1, 0, 0), (0, 0, 8)])
@@ -87,7 +87,7 @@ def test_ntt8_plan_partial_np(ck, ctx24, oracle_rns24, np3):
     try:
         check_ntt_plan(ck, ctx24, oracle_rns24, np3, 3, SEED_BASE + 3 * 1000 + 40 + sum(np3))
     finally:
-        ctx24.set_plan_forms(ntt="valu")
+        ctx24.set_plan_forms(ntt="int8")
 
 
 def test_ntt8_plan_boot40_terminal_primes(ck, ctx40, oracle_boot40):
@@ -97,7 +97,7 @@ def test_ntt8_plan_boot40_terminal_primes(ck, ctx40, oracle_boot40):
         check_ntt_plan(ck, ctx40, oracle_boot40, (43, 4, 12), 2, SEED_BASE + 3 * 1000 + 60,
                        mont=True)
     finally:
-        ctx40.set_plan_forms(ntt="valu", ntt8_blocks=16)
+        ctx40.set_plan_forms(ntt="int8", ntt8_blocks=16)
 
 
 def test_ntt8_plan_config3_batch(ck, ctx24, oracle_rns24):
@@ -110,7 +110,7 @@ def test_ntt8_plan_config3_batch(ck, ctx24, oracle_rns24):
         check_ntt_plan(ck, ctx24, oracle_rns24, (24, 0, 8), 64, SEED_BASE + 3 * 1000 + 100,
                        check=sample)
     finally:
-        ctx24.set_plan_forms(ntt="valu")
+        ctx24.set_plan_forms(ntt="int8")
 
 
 def test_ntt8_autotune(ck, ctx24, oracle_rns24):
@@ -124,7 +124,7 @@ def test_ntt8_autotune(ck, ctx24, oracle_rns24):
         best = min(tuning, key=lambda t: t[3])
         assert plan.info()["form"] == best[0]
     finally:
-        ctx24.set_plan_forms(ntt="valu")
+        ctx24.set_plan_forms(ntt="int8")
 
 
 def centred_sources(P, src_p, n, seed):
@@ -155,7 +155,7 @@ def check_bconv_plan(ck, ctx, P, src_p, dst_p, n, seed, check=None):
     return plan
 
 
-@pytest.mark.parametrize("form", ["int8", "valu"])
+@pytest.mark.parametrize("form", ["int8", "fp64", "valu"])
 @pytest.mark.parametrize("npoly", [1, 6])
 def test_bconv_plan_forms_8_to_32(ck, ctx24, oracle_rns24, form, npoly):
     ctx24.set_plan_forms(bconv=form)
@@ -164,10 +164,10 @@ def test_bconv_plan_forms_8_to_32(ck, ctx24, oracle_rns24, form, npoly):
                                 npoly, SEED_BASE + 3 * 1000 + 300 + npoly)
         assert plan.info()["form"] == ck.BCONV_FORMS[form]
     finally:
-        ctx24.set_plan_forms(bconv="fp64")
+        ctx24.set_plan_forms(bconv="int8")
 
 
-@pytest.mark.parametrize("form", ["int8", "valu"])
+@pytest.mark.parametrize("form", ["int8", "fp64", "valu"])
 def test_bconv_plan_forms_12_to_20(ck, ctx24, oracle_rns24, form):
     """two int8 k-steps (12 sources), a partly filled second row tile"""
     ctx24.set_plan_forms(bconv=form)
@@ -175,10 +175,10 @@ def test_bconv_plan_forms_12_to_20(ck, ctx24, oracle_rns24, form):
         check_bconv_plan(ck, ctx24, oracle_rns24, list(range(20, 32)), list(range(20)), 2,
                          SEED_BASE + 3 * 1000 + 320)
     finally:
-        ctx24.set_plan_forms(bconv="fp64")
+        ctx24.set_plan_forms(bconv="int8")
 
 
-@pytest.mark.parametrize("form", ["int8", "valu"])
+@pytest.mark.parametrize("form", ["int8", "fp64", "valu"])
 def test_bconv_plan_forms_16_to_43(ck, ctx40, oracle_boot40, form):
     """16 sources, 43 destinations (three row tiles) of bootparam_40"""
     ctx40.set_plan_forms(bconv=form)
@@ -187,7 +187,7 @@ def test_bconv_plan_forms_16_to_43(ck, ctx40, oracle_boot40, form):
         dst = [i for i in range(59) if i not in src]
         check_bconv_plan(ck, ctx40, oracle_boot40, src, dst, 2, SEED_BASE + 3 * 1000 + 340)
     finally:
-        ctx40.set_plan_forms(bconv="fp64")
+        ctx40.set_plan_forms(bconv="int8")
 
 
 def test_bconv_plan_int8_config3_batch(ck, ctx24, oracle_rns24):
@@ -199,7 +199,7 @@ def test_bconv_plan_int8_config3_batch(ck, ctx24, oracle_rns24):
         check_bconv_plan(ck, ctx24, oracle_rns24, list(range(24, 32)), list(range(32)), 64,
                          SEED_BASE + 3 * 1000 + 360, check=sample)
     finally:
-        ctx24.set_plan_forms(bconv="fp64")
+        ctx24.set_plan_forms(bconv="int8")
 
 
 def test_bconv_autotune(ck, ctx24, oracle_rns24):
@@ -211,4 +211,4 @@ def test_bconv_autotune(ck, ctx24, oracle_rns24):
         assert sorted(t[0] for t in tuning) == [0, 1, 2]
         assert plan.info()["form"] == min(tuning, key=lambda t: t[3])[0]
     finally:
-        ctx24.set_plan_forms(bconv="fp64")
+        ctx24.set_plan_forms(bconv="int8")

@@ -24,14 +24,15 @@ def ck():
 
 
 # every test runs with BaseConv fused into the forward NTT column pass, with
-# BaseConv on the FP64 matrix cores, and (BaseConv fused) with the key inner
-# product as its own accumulation pass instead of fused with the ModUp's row
-# stages: outputs must be identical
-CONFIGS = ["bconv-fused", "bconv-mfma", "key-accum-pass"]
+# BaseConv on the FP64 matrix cores, with BaseConv on the int8 matrix cores,
+# and (BaseConv fused) with the key inner product as its own accumulation
+# pass instead of fused with the ModUp's row stages: outputs must be
+# identical
+CONFIGS = ["bconv-fused", "bconv-mfma", "bconv-int8", "key-accum-pass"]
 
 
 def configure(ctx, name):
-    ctx.set_bconv_mfma(name == "bconv-mfma")
+    ctx.set_bconv_mfma("int8" if name == "bconv-int8" else name == "bconv-mfma")
     ctx.set_fused_key_rows(name != "key-accum-pass")
     return ctx
 
