@@ -301,6 +301,25 @@ __global__ __launch_bounds__(kThreads) void tensor_kernel(
   st4(out.ptr[2] + o, make_uint4(aa[0], aa[1], aa[2], aa[3]));
 }
 
+// TensorSquare (ElementWise.cu:549-575): the Tensor of a ciphertext with
+// itself reads its two polynomials once: (b^2, 2 b a, a^2), Montgomery
+// products as in tensor_kernel (2 b a equals the Karatsuba middle term
+// (b + a)^2 - b^2 - a^2 as a canonical residue).
+__global__ __launch_bounds__(kThreads) void tensor_square_kernel(
+    const EwNp np, const PrimeTab P, EwOperand c1, EwOperand out) {
+  EW_PROLOGUE
+  const uint4 b = ld4(c1.ptr[0] + ew_off(c1, np, y, k));
+  const uint4 a = ld4(c1.ptr[1] + ew_off(c1, np, y, k));
+  const uint4 bb = map2(b, b, [&](uint32_t u, uint32_t v) { return mont_mul(u, v, q, qinv); });
+  const uint4 ba = map2(b, a, [&](uint32_t u, uint32_t v) { return mont_mul(u, v, q, qinv); });
+  const uint4 aa = map2(a, a, [&](uint32_t u, uint32_t v) { return mont_mul(u, v, q, qinv); });
+  const uint4 mid = map2(ba, ba, [&](uint32_t u, uint32_t v) { return add_mod(u, v, q); });
+  const size_t o = ew_off(out, np, y, k);
+  st4(out.ptr[0] + o, bb);
+  st4(out.ptr[1] + o, mid);
+  st4(out.ptr[2] + o, aa);
+}
+
 // Key inner product of MultKeyNoModDown (Context.cpp:609-712 ->
 // ElementWise::PAccum, CPAccum<2,false>) fused with the P * (b, a) fold
 // (CAccum, CPAccumAdd<2,true>):
@@ -663,8 +682,13 @@ void launch_crt_lift(const Context &ctx, const uint32_t *src, const LiftArgs &A,
 void launch_tensor(const Context &ctx, const EwNp &np, const EwOperand &c1,
                    const EwOperand &c2, const EwOperand &out, hipStream_t st) {
   if (np.num_total <= 0) return;
-  launch(ctx, "tensor", np.num_total, tensor_kernel, ew_grid(np),
-         dim3(kThreads), st, np, ctx.ptab, c1, c2, out);
+  // the same ciphertext twice: the squaring form (ElementWise.cu:940-945)
+  if (c1.ptr[0] == c2.ptr[0] && c1.ptr[1] == c2.ptr[1] && c1.gap == c2.gap)
+    launch(ctx, "tensor_square", np.num_total, tensor_square_kernel, ew_grid(np),
+           dim3(kThreads), st, np, ctx.ptab, c1, out);
+  else
+    launch(ctx, "tensor", np.num_total, tensor_kernel, ew_grid(np),
+           dim3(kThreads), st, np, ctx.ptab, c1, c2, out);
 }
 
 void launch_gs_accum(const Context &ctx, const EwNp &np, const GsAccumArgs &A,

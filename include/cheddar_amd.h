@@ -210,7 +210,9 @@ int ck_caccum(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly
               int nterms, uint32_t *const *dst, const uint32_t *const *x,
               const uint32_t *const *consts, int accumulate, void *stream);
 /* ElementWiseHandler::Tensor (ElementWise.cu:921-953, kernel :504-546):
- * (d0, d1, d2) = (b1 b2, b1 a2 + a1 b2, a1 a2) on num_q limbs */
+ * (d0, d1, d2) = (b1 b2, b1 a2 + a1 b2, a1 a2) on num_q limbs; with
+ * b1 == b2 and a1 == a2 (the same ciphertext) the TensorSquare form
+ * (ElementWise.cu:549-575) reads the two polynomials once. */
 int ck_tensor(ck_context *ctx, int num_main, int num_ter, uint32_t *d0,
               uint32_t *d1, uint32_t *d2, const uint32_t *b1,
               const uint32_t *a1, const uint32_t *b2, const uint32_t *a2,

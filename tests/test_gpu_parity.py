@@ -384,6 +384,25 @@ def test_tensor(ck, ctx24, oracle_rns24):
         assert_same(ck.to_host(outs[i]), want[i], f"tensor d{i}")
 
 
+def test_tensor_square(ck, ctx24, oracle_rns24):
+    """TensorSquare (ElementWise.cu:549-575): the same ciphertext as both
+    operands (the squaring kernel) equals the oracle's general Tensor of the
+    ciphertext with a copy of itself, including 0 and q - 1 coefficients"""
+    P = oracle_rns24
+    pidx = P.level_primes(23)
+    b, a = (uniform(P, pidx, SEED_BASE + 75 + i) for i in range(2))
+    q = np.array([P.prime(i) for i in pidx], dtype=np.uint32)
+    b[:, :2] = np.stack([np.zeros_like(q), q - 1], axis=1)
+    a[:, :2] = np.stack([q - 1, q - 1], axis=1)
+    want = P.tensor(b, a, b.copy(), a.copy(), pidx)
+    db, da = ck.to_device(b), ck.to_device(a)
+    outs = [torch.empty_like(db) for _ in range(3)]
+    ctx24.tensor(*outs, db, da, db, da, 24, 0)
+    sync()
+    for i in range(3):
+        assert_same(ck.to_host(outs[i]), want[i], f"tensor square d{i}")
+
+
 @pytest.mark.parametrize("rot", [1, 1234, 32767, -1])
 def test_permute(ck, ctx24, oracle_rns24, rot):
     P = oracle_rns24
