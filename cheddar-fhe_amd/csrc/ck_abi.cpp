@@ -863,7 +863,7 @@ int ck_plan_bconv(ck_context *ctx, const int *src_primes, int n_src,
   CK_REQUIRE(ctx && ctx->c && src_primes && dst_primes && src && dst, "null argument");
   const Context &c = *ctx->c;
   CK_REQUIRE(n >= 1, "a plan needs at least one polynomial");
-  CK_REQUIRE(n_src >= 1 && n_src <= kMaxBconvSrc, "n_src must be 1..16");
+  CK_REQUIRE(n_src >= 1 && n_src <= kMaxBconvSrc, "n_src must be 1..24");
   CK_REQUIRE(n_dst >= 1 && n_dst <= kMaxMfmaDst, "n_dst must be 1..64");
   for (int i = 0; i < n_src; i++) {
     CK_REQUIRE(src_primes[i] >= 0 && src_primes[i] < c.num_primes, "invalid source prime");

@@ -5,7 +5,7 @@
 // over S centred int32 source limbs v_k (|v| < 2^30). As a matrix product
 // the rows are (destination, half) pairs, the reduction runs over the
 // sources and the columns over positions; with 16-bit table halves every
-// product is below 2^46 and every sum of S <= 16 products below 2^50, so
+// product is below 2^46 and every sum of S <= 24 products below 2^51, so
 // v_mfma_f64_16x16x4_f64 computes the two partial sums exactly. The vector
 // unit only recombines hi * 2^16 + lo and reduces (one exact modular
 // reduction per output), instead of 2 S FMAs per output in the fused
@@ -147,8 +147,10 @@ void pick_ks(int max_s, F &&f) {
     f(bconv_mfma_kernel<RT, 2>, bconv_mfma_table_kernel<RT, 2>);
   else if (ks == 3)
     f(bconv_mfma_kernel<RT, 3>, bconv_mfma_table_kernel<RT, 3>);
-  else
+  else if (ks == 4)
     f(bconv_mfma_kernel<RT, 4>, bconv_mfma_table_kernel<RT, 4>);
+  else
+    f(bconv_mfma_kernel<RT, kMaxMfmaKSteps>, bconv_mfma_table_kernel<RT, kMaxMfmaKSteps>);
 }
 
 template <class F>

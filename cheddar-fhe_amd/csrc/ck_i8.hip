@@ -338,7 +338,8 @@ typedef const __attribute__((address_space(4))) BconvI8Inst CBInst;
 
 // int8 form: workgroup = 256 positions of one instance (64 per wave, four
 // 16-position tiles). Rows = 16 destinations per tile, K = 8 sources x 4
-// digits per v_mfma_i32_16x16x32_i8 (KS k-steps for up to 8 KS sources).
+// digits per v_mfma_i32_16x16x32_i8 (KS <= 3 k-steps for up to 8 KS
+// sources: each plane sums at most 96 digit products, |D_j| < 2^21).
 // Source indices past S are clamped to 0 (their plane bytes are 0), so every
 // load is unconditional.
 template <int KS, class Inst>
@@ -426,7 +427,7 @@ __global__ __launch_bounds__(256) void bconv_i8_arg_kernel(const BconvI8Launch L
 
 // Vector-unit form: one position per thread, the table row of each
 // destination from scalar loads; per destination sum_i v_i hi_i and
-// sum_i v_i lo_i in exact FP64 (|v| < 2^31, halves < 2^16, S <= 16 terms:
+// sum_i v_i lo_i in exact FP64 (|v| < 2^30, halves < 2^16, S <= 24 terms:
 // below 2^51), then hi 2^16 + lo reduced as in the fused column pass.
 __global__ __launch_bounds__(256) void bconv_valu_kernel(const BconvI8Inst *__restrict__ insts,
                                                          PrimeTab P) {
@@ -574,8 +575,11 @@ void launch_bconv_i8_table(const Context &c, const BconvI8Inst *insts, int n, in
   if (max_s <= 8)
     launch(c, "bconv_i8", dsts, bconv_i8_kernel<1>, dim3(kN / 256, n), dim3(256), st, insts,
            c.ptab);
-  else
+  else if (max_s <= 16)
     launch(c, "bconv_i8", dsts, bconv_i8_kernel<2>, dim3(kN / 256, n), dim3(256), st, insts,
+           c.ptab);
+  else
+    launch(c, "bconv_i8", dsts, bconv_i8_kernel<3>, dim3(kN / 256, n), dim3(256), st, insts,
            c.ptab);
 }
 
@@ -589,8 +593,11 @@ void launch_bconv_i8(const Context &c, const BconvI8Launch &L, hipStream_t st) {
   if (max_s <= 8)
     launch(c, "bconv_i8", dsts, bconv_i8_arg_kernel<1>, dim3(kN / 256, L.n), dim3(256), st,
            L, c.ptab);
-  else
+  else if (max_s <= 16)
     launch(c, "bconv_i8", dsts, bconv_i8_arg_kernel<2>, dim3(kN / 256, L.n), dim3(256), st,
+           L, c.ptab);
+  else
+    launch(c, "bconv_i8", dsts, bconv_i8_arg_kernel<3>, dim3(kN / 256, L.n), dim3(256), st,
            L, c.ptab);
 }
 

@@ -32,14 +32,14 @@ CK_HD int pack_c(int x) { return (int)(((uint32_t)x + 0x80808080u) ^ 0x80808080u
 // [-2^30, 2^30), inside the range above)
 CK_HD int pack_off(uint32_t x) { return (int)((x + 0x40808080u) ^ 0x80808080u); }
 
-// sum_j 2^(8 j) d_j from four int32 partial sums. Bounds (each D_j a sum
-// of at most 64 products of two int8 digits, |product| <= 2^14):
-// |d_1|, |d_2|, |d_3| <= 2^20 and |d_0| <= 2^20 + 2^30 (an offset
-// correction below 2^30 may start the d_0 accumulation). Then
-// |lo| = |d_0 + 2^8 d_1| <= 2^30 + 2^20 + 2^28 < 2^31 and
-// |hi| = |d_2 + 2^8 d_3| <= 2^20 + 2^28, so neither int32 sum wraps, and
-// hi 2^16 + lo (< 2^45 in magnitude) is an exact FP64 value: the fma
-// rounds nothing.
+// sum_j 2^(8 j) d_j from four int32 partial sums. Bounds: each D_j is a
+// sum of at most 96 products of two int8 digits (|product| <= 2^14), so
+// |d_1|, |d_2|, |d_3| <= 96 2^14 < 2^21, and |d_0| < 2^21 + 2^30 (an offset
+// correction below 2^30 may start the d_0 accumulation; it is only used
+// with 64 products). Then |lo| = |d_0 + 2^8 d_1| < 2^30 + 2^21 + 2^29
+// < 2^31 and |hi| = |d_2 + 2^8 d_3| < 2^21 + 2^29, so neither int32 sum
+// wraps, and hi 2^16 + lo (< 2^46 in magnitude) is an exact FP64 value:
+// the fma rounds nothing.
 CK_HD double combine(int d0, int d1, int d2, int d3) {
   const int lo = d0 + d1 * 256;
   const int hi = d2 + d3 * 256;

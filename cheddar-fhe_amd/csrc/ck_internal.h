@@ -98,7 +98,10 @@ struct LimbJob {
 };
 static_assert(sizeof(LimbJob) == 64, "LimbJob is one 64-byte record");
 
-constexpr int kMaxBconvSrc = 16;  // source limbs of one BaseConv
+// source limbs of one BaseConv: up to 5 terminal + 12 auxiliary primes in
+// the reference's parameter files (bootparam_35.json); 24 keeps every exact
+// FP64 and int8 sum within its stated bound
+constexpr int kMaxBconvSrc = 24;
 constexpr int kMaxBconvDst = kMaxLaunchLimbs;
 
 // Tables of one ModDown flavour (ModDown, Rescale, ModDownAndRescale) at one

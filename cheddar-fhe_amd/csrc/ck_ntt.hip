@@ -203,7 +203,8 @@ __device__ __forceinline__ ColTile col_tile() {
 // src/core/ModSwitch.cu:23-171, in exact FP64): for each coefficient,
 // y = sum_j v_j * T_j mod t over bs centred int32 source limbs, with
 // T_j = hi_j * 2^16 + lo_j so every partial sum stays an exact double
-// (|v| < 2^30: |sum v hi| < 2^49, |sum v lo| < 2^50 for bs <= 16).
+// (|v| < 2^30, hi < 2^15, lo < 2^16: |sum v hi| < 2^50 and
+// |sum v lo| < 2^51 for bs <= 24).
 // Output: int32 tile [256][16] in LDS, 16-byte source loads.
 //
 // MS (>= bs) bounds the unrolled source loop. All loads
@@ -271,10 +272,11 @@ __device__ __forceinline__ void bconv_cols_ms(const int32_t *bsrc, int bs,
 }
 
 // bs is uniform per workgroup: one branch-free body per source bound.
-// WIDE: the instantiation for launches with more than 9 sources, whose
+// WIDTH 1: the instantiation for launches with 10..16 sources, whose
 // 12- and 13-source bodies keep 16-byte loads at four waves per SIMD (at
-// five they only fit with 8-byte loads)
-template <bool WIDE>
+// five they only fit with 8-byte loads); WIDTH 2: 17..24 sources (terminal
+// plus auxiliary primes of a rescale), at three waves per SIMD
+template <int WIDTH>
 __device__ __forceinline__ void bconv_cols_to_lds(const int32_t *bsrc, int bs,
                                                   const double *tab, int c0,
                                                   const ModF &m,
@@ -285,13 +287,13 @@ __device__ __forceinline__ void bconv_cols_to_lds(const int32_t *bsrc, int bs,
     case 8: bconv_cols_ms<8, 4>(bsrc, bs, tab, c0, m, lds); return;
     case 9: bconv_cols_ms<9, 4, 5>(bsrc, bs, tab, c0, m, lds); return;
     case 12:
-      if (WIDE)
+      if (WIDTH >= 1)
         bconv_cols_ms<12, 4, 6>(bsrc, bs, tab, c0, m, lds);
       else
         bconv_cols_ms<12, 2>(bsrc, bs, tab, c0, m, lds);
       return;
     case 13:
-      if (WIDE)
+      if (WIDTH >= 1)
         bconv_cols_ms<13, 4, 5>(bsrc, bs, tab, c0, m, lds);
       else
         bconv_cols_ms<13, 2>(bsrc, bs, tab, c0, m, lds);
@@ -304,8 +306,10 @@ __device__ __forceinline__ void bconv_cols_to_lds(const int32_t *bsrc, int bs,
     bconv_cols_ms<8, 4>(bsrc, bs, tab, c0, m, lds);
   else if (bs <= 12)
     bconv_cols_ms<12, 2>(bsrc, bs, tab, c0, m, lds);
-  else
-    bconv_cols_ms<kMaxBconvSrc, 2>(bsrc, bs, tab, c0, m, lds);
+  else if (WIDTH < 2 || bs <= 16)
+    bconv_cols_ms<16, 2>(bsrc, bs, tab, c0, m, lds);
+  else if constexpr (WIDTH == 2)
+    bconv_cols_ms<kMaxBconvSrc, 2, 6>(bsrc, bs, tab, c0, m, lds);
 }
 
 // Final value of an element: canonical u32, or lazy int32 in
@@ -317,8 +321,8 @@ __device__ __forceinline__ int32_t lazy_i32(double x, const ModF &m) {
 // ---------------------------------------------------------------------------
 // Forward, column stages s = 0..7 -> lazy int32 intermediate
 // ---------------------------------------------------------------------------
-template <class D, bool WIDE>
-__global__ __launch_bounds__(kColThreads, WIDE ? 4 : 5) void ntt_fwd_cols(
+template <class D, int WIDTH>
+__global__ __launch_bounds__(kColThreads, 5 - WIDTH) void ntt_fwd_cols(
     const D desc, const PrimeTab P, const uint32_t *__restrict__ tw) {
   // int32 BaseConv tile [256][16] aliased with the double exchange tile
   __shared__ double ldsd[256 * kColTile];
@@ -348,7 +352,7 @@ __global__ __launch_bounds__(kColThreads, WIDE ? 4 : 5) void ntt_fwd_cols(
   double x[16];
   if (J.flags & kNttBconvInput) {
     // twiddles after the BaseConv, whose loads need the registers
-    bconv_cols_to_lds<WIDE>(J.bsrc, J.bs, J.btab, c0, m, lds);
+    bconv_cols_to_lds<WIDTH>(J.bsrc, J.bs, J.btab, c0, m, lds);
     load_twiddles();
     __syncthreads();
 #pragma unroll
@@ -834,11 +838,14 @@ const char *inv_rows_name(int flags) {
 template <class D>
 void fwd_passes(const Context &ctx, const D &d, int n, int flags, int max_bs,
                 hipStream_t st) {
-  if ((flags & kNttBconvInput) && max_bs > 9)
-    launch(ctx, fwd_cols_name(flags), n, ntt_fwd_cols<D, true>, dim3(kColTiles * n),
+  if ((flags & kNttBconvInput) && max_bs > 16)
+    launch(ctx, fwd_cols_name(flags), n, ntt_fwd_cols<D, 2>, dim3(kColTiles * n),
+           dim3(kColThreads), st, d, ctx.ptab, (const uint32_t *)ctx.d_tw_fwd);
+  else if ((flags & kNttBconvInput) && max_bs > 9)
+    launch(ctx, fwd_cols_name(flags), n, ntt_fwd_cols<D, 1>, dim3(kColTiles * n),
            dim3(kColThreads), st, d, ctx.ptab, (const uint32_t *)ctx.d_tw_fwd);
   else
-    launch(ctx, fwd_cols_name(flags), n, ntt_fwd_cols<D, false>, dim3(kColTiles * n),
+    launch(ctx, fwd_cols_name(flags), n, ntt_fwd_cols<D, 0>, dim3(kColTiles * n),
            dim3(kColThreads), st, d, ctx.ptab, (const uint32_t *)ctx.d_tw_fwd);
   if (!(flags & kNttColsOnly))
     launch(ctx, fwd_rows_name(flags), n, ntt_fwd_rows<D>, dim3(kRowTiles, n),

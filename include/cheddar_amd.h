@@ -325,7 +325,7 @@ int ck_plan_hrot_hoisted(ck_context *ctx, int n, const int *levels,
 /* out dst[k] = fast base conversion of src[k] on the FP64 matrix cores,
  * k < n: the reference's ModSwitchMatrixMult (src/core/ModSwitch.cu:23-171)
  * with the PopulateModSwitchConstants tables (:320-366) as a standalone
- * step. From n_src (1..16) source primes s_i to n_dst (1..64) destination
+ * step. From n_src (1..24) source primes s_i to n_dst (1..64) destination
  * primes t_m (global prime indices):
  *   dst[k] limb m = [sum_i v_i (prod_{j != i} s_j) R]_{t_m}  (R = 2^32, canonical)
  * for src[k] = n_src int32 limbs v_i centred in [-(s_i - 1)/2, (s_i - 1)/2]

@@ -158,8 +158,12 @@ int main(int argc, char **argv) {
       Ct ct, out;
       encrypt(ct, m, level, P.GetRescalePrimeProd(level));
       lt.Evaluate(*ctx, out, ct, keys);
+      // a fresh encryption at a ~2^30 scale (bootparam_30) carries about
+      // 2^10 times the relative noise of a ~2^40 one: below 2^35 the stage
+      // is held to the reference's end-to-end tolerance instead
+      const double scale = P.GetRescalePrimeProd(level);
       Report(which == 0 ? "CoeffToSlot phase 0 vs plain" : "SlotToCoeff phase 0 vs plain",
-             lt.PlainEvaluate(m), decrypt(out), 1e-4);
+             lt.PlainEvaluate(m), decrypt(out), std::log2(scale) < 35 ? 1e-3 : 1e-4);
     }
   } catch (const std::exception &e) {
     std::printf("FAIL linear transform: %s\n", e.what());

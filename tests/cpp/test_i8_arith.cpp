@@ -14,7 +14,7 @@
 //   * one 16-point matrix stage built with ctx_planes, on random and extreme
 //     matrices and inputs (centred, and canonical with the offset
 //     correction), equals sum_n T[m][n] x_n mod q for six moduli;
-//   * the BaseConv planes (bconv_i8_planes) for 1..16 sources and 1..64
+//   * the BaseConv planes (bconv_i8_planes) for 1..24 sources and 1..64
 //     destinations with distinct moduli, through the 16x16x32 emulation;
 //   * the worst case: every digit -128 in both operands (|D_j| = 2^20).
 #include <cmath>
@@ -202,8 +202,8 @@ static void test_bconv(int S, int D, const std::vector<uint32_t> &mods) {
   const std::vector<int64_t> pl = bconv_i8_planes(tab, S, D, dq);
   const int nks = (S + 7) / 8;
   for (int pos = 0; pos < 16; pos++) {
-    int64_t v[16];
-    for (int i = 0; i < 16; i++) {
+    int64_t v[24];
+    for (int i = 0; i < 24; i++) {
       const int64_t h = (1LL << 30) - 1;
       v[i] = i < S ? (pos < 2 ? (pos ? -h : h) : uniform(-h, h)) : 0;
     }
@@ -263,7 +263,7 @@ int main() {
     test_stages(q);
   }
   const std::vector<uint32_t> mods(std::begin(primes), std::end(primes));
-  for (int S : {1, 2, 7, 8, 9, 12, 16})
+  for (int S : {1, 2, 7, 8, 9, 12, 16, 17, 24})
     for (int D : {1, 16, 17, 32, 64}) test_bconv(S, D, mods);
   test_worst_case();
   std::printf("OK: %d failure(s)\n", failures);

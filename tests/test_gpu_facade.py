@@ -24,6 +24,10 @@ BOOT = os.path.join(REPO, "tests", "cpp", "test_boot")
     (os.path.join(PKG, "params", "rns24_aux8.json"), 23),
     (os.path.join(PKG, "params", "rns24_aux8.json"), 6),
     (os.path.join(GOLDEN, "params", "bootparam_40.json"), 20),
+    (os.path.join(GOLDEN, "params", "bootparam_30.json"), 34),
+    (os.path.join(GOLDEN, "params", "bootparam_30.json"), 3),
+    (os.path.join(GOLDEN, "params", "bootparam_35.json"), 31),
+    (os.path.join(GOLDEN, "params", "bootparam_35.json"), 9),
 ])
 def test_cpp_facade(params, level):
     assert os.path.exists(BIN), "tests/cpp/test_context is not built (run build())"
@@ -38,10 +42,13 @@ def test_cpp_facade(params, level):
 
 # Encrypted semantics (unittest/BasicTest.cpp): encode, encrypt, evaluate,
 # decrypt, decode and compare with the plain computation (max error 1e-3).
-# bootparam_40 is run with a Hamming-weight-32 main secret (no sparse-secret
-# encapsulation), the setting the bootstrapping tests use.
+# The reference's parameter files (unittest/BasicTest.cpp:421-429 runs
+# bootparam_30/35/40) are run with a Hamming-weight-32 main secret (no
+# sparse-secret encapsulation), the setting the bootstrapping tests use.
 @pytest.mark.parametrize("params,weight", [
     (os.path.join(PKG, "params", "rns24_aux8.json"), None),
+    (os.path.join(GOLDEN, "params", "bootparam_30.json"), 32),
+    (os.path.join(GOLDEN, "params", "bootparam_35.json"), 32),
     (os.path.join(GOLDEN, "params", "bootparam_40.json"), 32),
 ])
 def test_cpp_basic_semantics(params, weight):
@@ -54,16 +61,19 @@ def test_cpp_basic_semantics(params, weight):
     assert "OK: 0 failure(s)" in r.stdout
 
 
-# Bootstrapping (unittest/Bootstrapping.cpp): bootparam_40, full slots,
-# message uniform in the unit box, max error 1e-3 as the reference's test;
-# stage checks (EvalMod, first CoeffToSlot / SlotToCoeff phases) inside.
-# "sse": the file's dense secret (Hamming weight 32768) with sparse-secret
-# encapsulation; "dense=sparse": a weight-32 main secret, no encapsulation.
+# Bootstrapping (unittest/Bootstrapping.cpp, instantiated on
+# bootparam_30/35/40 at :43-46): full slots, message uniform in the unit
+# box, max error 1e-3 as the reference's test; stage checks (EvalMod, first
+# CoeffToSlot / SlotToCoeff phases) inside. "sse": the file's dense secret
+# (Hamming weight 32768) with sparse-secret encapsulation; "dense=sparse": a
+# weight-32 main secret, no encapsulation.
+@pytest.mark.parametrize("params", ["bootparam_30.json", "bootparam_35.json",
+                                    "bootparam_40.json"])
 @pytest.mark.parametrize("mode", ["sse", "dense=sparse"])
-def test_cpp_bootstrap(mode):
+def test_cpp_bootstrap(mode, params):
     assert os.path.exists(BOOT), "tests/cpp/test_boot is not built (run build())"
-    print(f"bootstrap bootparam_40 {mode}", flush=True)
-    args = [BOOT, os.path.join(GOLDEN, "params", "bootparam_40.json"), "1"]
+    print(f"bootstrap {params} {mode}", flush=True)
+    args = [BOOT, os.path.join(GOLDEN, "params", params), "1"]
     if mode == "sse":
         args.append("sse")
     r = subprocess.run(args, capture_output=True, text=True, timeout=600)

@@ -4,7 +4,8 @@ ring dimension N = 2^16.
 
 Parameter sets: the 24 + 8 prime chain of the plain key-switching workload
 (q_i = 1 mod 2^17, largest primes below 2^31) and the reference's
-bootparam_40.json (terminal primes, 43 main, 12 auxiliary).
+bootparam_40.json (4 terminal, 43 main, 12 auxiliary primes),
+bootparam_35.json (5 terminal, 43 main) and bootparam_30.json (48 main).
 """
 import numpy as np
 import pytest
@@ -629,3 +630,74 @@ def test_boot40_hoisted_rotations(ck, ctx40, oracle_boot40):
         want = P.hrot(level, ct, k, r)
         assert_same(ck.to_host(o[0]), want[0], f"boot40 hoisted b rot {r}")
         assert_same(ck.to_host(o[1]), want[1], f"boot40 hoisted a rot {r}")
+
+
+# ---------------------------------------------------------------------------
+# bootparam_30.json (48 main primes, no terminal primes) and bootparam_35.json
+# (5 terminal primes: rescales drop up to 5 terminal primes and restore up to
+# 3 main ones, BaseConv from 17 sources); the reference's BasicTest runs on
+# both (unittest/BasicTest.cpp:421-429)
+# ---------------------------------------------------------------------------
+OTHER_PARAMS = {
+    "bootparam_30.json": {"modup": [34, 12, 0], "hmult": [34, 10],
+                          "moddown": [(34, 0), (34, 2), (23, 2), (5, 1), (1, 2)]},
+    "bootparam_35.json": {"modup": [31, 9, 3], "hmult": [31, 9],
+                          "moddown": [(31, 0), (31, 2), (9, 1), (9, 2), (3, 2), (2, 1),
+                                      (2, 2), (8, 2), (1, 2)]},
+}
+
+
+@pytest.fixture(scope="module", params=[(p, c) for p in OTHER_PARAMS
+                                        for c in ("bconv-fused", "bconv-int8")],
+                ids=lambda v: f"{v[0][:-5]}-{v[1]}")
+def other(ck, request):
+    import os
+    from conftest import load_params
+    import ckks_oracle as O
+    name, cfg = request.param
+    prm = load_params(name)
+    O.set_threads(min(8, os.cpu_count() or 1))
+    return name, configure(ck.Context.from_json(prm), cfg), O.Param.from_json(prm)
+
+
+def test_other_params_modup(ck, other):
+    name, ctx, P = other
+    for level in OTHER_PARAMS[name]["modup"]:
+        x = uniform(P, P.level_primes(level), SEED_BASE + 250 + level)
+        want = P.modup(x, level)
+        beta, tot = want.shape[0], want.shape[1]
+        out = torch.zeros((beta * tot, P.n), dtype=torch.int32, device="cuda")
+        ctx.modup(out, ck.to_device(x), level)
+        sync()
+        assert_same(ck.to_host(out).reshape(beta, tot, P.n), want, f"{name} modup {level}")
+
+
+def test_other_params_moddown(ck, other):
+    name, ctx, P = other
+    for level, kind in OTHER_PARAMS[name]["moddown"]:
+        srcs = [uniform(P, P.level_primes(level, with_aux=(kind != 1)),
+                        SEED_BASE + 350 + 2 * j + level) for j in range(2)]
+        want = [P.moddown(s, level, kind) for s in srcs]
+        outs = [torch.empty((w.shape[0], P.n), dtype=torch.int32, device="cuda") for w in want]
+        ctx.moddown(outs, [ck.to_device(s) for s in srcs], level, kind)
+        sync()
+        for j in range(2):
+            assert_same(ck.to_host(outs[j]), want[j], f"{name} moddown kind {kind} level {level}")
+
+
+def test_other_params_hmult(ck, other):
+    name, ctx, P = other
+    key = full_key(P, SEED_BASE + 2500)
+    dk = dev_key(*key)
+    for level in OTHER_PARAMS[name]["hmult"]:
+        pidx = P.level_primes(level)
+        ct1 = [uniform(P, pidx, SEED_BASE + 450 + i + level) for i in range(2)]
+        ct2 = [uniform(P, pidx, SEED_BASE + 460 + i + level) for i in range(2)]
+        want = P.hmult(level, ct1, ct2, key, rescale=True)
+        out = [torch.empty((want[0].shape[0], P.n), dtype=torch.int32, device="cuda")
+               for _ in range(2)]
+        ctx.hmult(out, [ck.to_device(v) for v in ct1], [ck.to_device(v) for v in ct2], dk,
+                  level, rescale=True)
+        sync()
+        for j in range(2):
+            assert_same(ck.to_host(out[j]), want[j], f"{name} hmult level {level} poly {j}")
