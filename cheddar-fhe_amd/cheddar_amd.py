@@ -42,6 +42,7 @@ _SIGS = {
     "ck_level_to_np": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip]),
     "ck_np_to_level": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "ck_modup_beta": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "ck_context_set_short_base": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "ck_galois_factor": (ctypes.c_int, [_vp, ctypes.c_int, _u32p]),
     "ck_prime": (ctypes.c_int, [_vp, ctypes.c_int, _u32p]),
     "ck_ntt": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -253,6 +254,12 @@ class Context:
 
     def modup_beta(self, level):
         return lib.ck_modup_beta(self._h, level)
+
+    def set_short_base(self, num_main, num_ter):
+        """level -1 of sparse-secret encapsulation: the short base's key
+        switching with as many auxiliary primes (key blocks: the short base's
+        limbs, then its auxiliary limbs)"""
+        _check(lib.ck_context_set_short_base(self._h, num_main, num_ter))
 
     def galois_factor(self, rot):
         g = ctypes.c_uint32()

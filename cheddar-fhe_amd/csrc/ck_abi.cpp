@@ -88,7 +88,7 @@ bool valid_ptrs(const uint32_t *const *p, int n) {
 
 bool valid_key(const Context &c, const uint32_t *const *k, int level) {
   if (!k) return false;
-  const LevelPlan &lp = c.levels[level];
+  const LevelPlan &lp = level_plan(c, level);
   for (int g = 0; g < lp.beta; g++)
     if (lp.groups[g].s1 > lp.groups[g].s0 && !k[g]) return false;
   return true;
@@ -174,9 +174,10 @@ int ck_context_info(const ck_context *ctx, int *out) {
 int ck_level_to_np(const ck_context *ctx, int level, int *num_main,
                    int *num_ter) {
   CK_REQUIRE(ctx && ctx->c && num_main && num_ter, "null argument");
-  CK_REQUIRE(valid_level(*ctx->c, level), "invalid level");
-  *num_main = ctx->c->lv_main[level];
-  *num_ter = ctx->c->lv_ter[level];
+  CK_REQUIRE(has_switch_level(*ctx->c, level), "invalid level");
+  const LevelPlan &lp = level_plan(*ctx->c, level);
+  *num_main = lp.nm;
+  *num_ter = lp.nt;
   return CK_OK;
 }
 
@@ -188,9 +189,14 @@ int ck_np_to_level(const ck_context *ctx, int num_main, int num_ter) {
   return -1;
 }
 
+int ck_context_set_short_base(ck_context *ctx, int num_main, int num_ter) {
+  CK_REQUIRE(ctx && ctx->c, "null context");
+  return build_short_base(*ctx->c, num_main, num_ter);
+}
+
 int ck_modup_beta(const ck_context *ctx, int level) {
-  if (!ctx || !ctx->c || !valid_level(*ctx->c, level)) return -1;
-  return ctx->c->levels[level].beta;
+  if (!ctx || !ctx->c || !has_switch_level(*ctx->c, level)) return -1;
+  return level_plan(*ctx->c, level).beta;
 }
 
 int ck_galois_factor(const ck_context *ctx, int rot, uint32_t *out) {
@@ -252,7 +258,7 @@ int ck_modup(ck_context *ctx, int level, uint32_t *dst, const uint32_t *src,
              void *stream) {
   CK_REQUIRE(ctx && ctx->c && dst && src, "null argument");
   const Context &c = *ctx->c;
-  CK_REQUIRE(valid_level(c, level), "invalid level");
+  CK_REQUIRE(has_switch_level(c, level), "invalid level");
   CK_REQUIRE(dst != src, "ModUp is not in place");
   ImmediateSink sink(c, as_stream(stream));
   return op_modup(c, level, dst, src, c.ws, sink);
@@ -263,7 +269,7 @@ int ck_moddown(ck_context *ctx, int level, int kind, int npoly,
                void *stream) {
   CK_REQUIRE(ctx && ctx->c, "null context");
   const Context &c = *ctx->c;
-  CK_REQUIRE(valid_level(c, level), "invalid level");
+  CK_REQUIRE(has_switch_level(c, level), "invalid level");
   CK_REQUIRE(kind >= 0 && kind <= 2, "invalid ModDown kind");
   CK_REQUIRE(npoly >= 1 && npoly <= 4, "npoly must be 1..4");
   CK_REQUIRE(valid_ptrs(dst, npoly) && valid_ptrs(src, npoly), "null pointer");
@@ -570,7 +576,7 @@ int ck_mult_key_no_moddown(ck_context *ctx, int level, uint32_t *accum_b,
                            const uint32_t *const *key_a, void *stream) {
   CK_REQUIRE(ctx && ctx->c && accum_b && accum_a && x, "null argument");
   const Context &c = *ctx->c;
-  CK_REQUIRE(valid_level(c, level), "invalid level");
+  CK_REQUIRE(has_switch_level(c, level), "invalid level");
   CK_REQUIRE(valid_key(c, key_b, level) && valid_key(c, key_a, level),
              "null key block");
   CK_REQUIRE(accum_b != x && accum_a != x,
@@ -586,7 +592,7 @@ int ck_mult_key(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
                 void *stream) {
   CK_REQUIRE(ctx && ctx->c && out_b && out_a && b && a, "null argument");
   const Context &c = *ctx->c;
-  CK_REQUIRE(valid_level(c, level), "invalid level");
+  CK_REQUIRE(has_switch_level(c, level), "invalid level");
   CK_REQUIRE(valid_key(c, key_b, level) && valid_key(c, key_a, level),
              "null key block");
   ImmediateSink sink(c, as_stream(stream));

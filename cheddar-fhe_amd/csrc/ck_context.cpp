@@ -166,21 +166,15 @@ void bconv_constants(const Context &c, const std::vector<int> &src,
 
 namespace {
 
-// One ModDown flavour at one level (ModSwitch.cu:233-303 for the prime
-// ranges, :566-684 for which limbs are transformed, NTT.cu:456-601 for the
-// epilogue indexing).
-int build_moddown(Context &c, int level, int kind, ModDownPlan &mp) {
-  const int alpha = c.alpha;
-  const int nm = c.lv_main[level], nt = c.lv_ter[level];
+// One ModDown flavour from (nm, nt) q primes with naux auxiliary primes to
+// (dm, dt) (ModSwitch.cu:233-303 for the prime ranges, :566-684 for which
+// limbs are transformed, NTT.cu:456-601 for the epilogue indexing). kind 0
+// (ModDown) has dm = nm, dt = nt.
+int build_moddown(Context &c, int nm, int nt, int dm, int dt, int naux, int kind,
+                  ModDownPlan &mp) {
   const int num_q = nm + nt;
-  int dm = nm, dt = nt;
-  if (kind != 0) {
-    if (level == 0) return kOk;  // no rescale below level 0
-    dm = c.lv_main[level - 1];
-    dt = c.lv_ter[level - 1];
-  }
   const int next_num_q = dm + dt;
-  const int num_src_aux = (kind == 1) ? 0 : alpha;
+  const int num_src_aux = (kind == 1) ? 0 : naux;
   std::vector<int> dst_prime(next_num_q);
   for (int y = 0; y < next_num_q; y++) dst_prime[y] = c.max_ter - dt + y;
 
@@ -245,33 +239,34 @@ int build_moddown(Context &c, int level, int kind, ModDownPlan &mp) {
   return kOk;
 }
 
-int build_level(Context &c, int level) {
-  LevelPlan &lp = c.levels[level];
-  const int alpha = c.alpha;
-  lp.nm = c.lv_main[level];
-  lp.nt = c.lv_ter[level];
-  lp.num_q = lp.nm + lp.nt;
-  lp.pad = c.max_ter - lp.nt;
-  lp.beta = (lp.nm + c.max_ter + alpha - 1) / alpha;  // ModSwitch.cu:181-185
+// ModUp groups of naux q limbs each after `pad` padding limbs (the
+// ModSwitchHandler constructor, ModSwitch.cu:175-232), for the q primes
+// (nm, nt) and the first naux auxiliary primes.
+int build_modup(Context &c, LevelPlan &lp, int nm, int nt, int naux, int pad) {
+  lp.nm = nm;
+  lp.nt = nt;
+  lp.num_q = nm + nt;
+  lp.pad = pad;
+  const int padded = lp.num_q + pad;
+  lp.beta = (padded + naux - 1) / naux;
   lp.q_prime.resize(lp.num_q);
-  for (int t = 0; t < lp.num_q; t++) lp.q_prime[t] = lp.pad + t;
+  for (int t = 0; t < lp.num_q; t++) lp.q_prime[t] = c.max_ter - nt + t;
   lp.modup_c1.assign(lp.num_q, 0);
   lp.groups.assign(lp.beta, ModUpGroup());
-  const int padded = lp.num_q + lp.pad;
   for (int g = 0; g < lp.beta; g++) {
-    int s0 = g * alpha, s1 = std::min((g + 1) * alpha, padded);
+    int s0 = g * naux, s1 = std::min((g + 1) * naux, padded);
     ModUpGroup &gr = lp.groups[g];
-    if (s1 <= lp.pad) {
+    if (s1 <= pad) {
       gr.s0 = gr.s1 = 0;
       continue;
     }
-    s0 = std::max(0, s0 - lp.pad);
-    s1 -= lp.pad;
+    s0 = std::max(0, s0 - pad);
+    s1 -= pad;
     gr.s0 = s0;
     gr.s1 = s1;
     std::vector<int> src, dst;
     for (int t = s0; t < s1; t++) src.push_back(lp.q_prime[t]);
-    for (int t = 0; t < lp.num_q + alpha; t++) {
+    for (int t = 0; t < lp.num_q + naux; t++) {
       if (t >= s0 && t < s1) continue;
       gr.dst_limb.push_back(t);
       dst.push_back(t < lp.num_q ? lp.q_prime[t] : c.L + (t - lp.num_q));
@@ -289,9 +284,20 @@ int build_level(Context &c, int level) {
     int rc = upload(c, &gr.d_ftab, ftab);
     if (rc) return rc;
   }
-  int rc = build_moddown(c, level, 0, lp.moddown);
-  if (!rc) rc = build_moddown(c, level, 1, lp.rescale);
-  if (!rc) rc = build_moddown(c, level, 2, lp.moddown_rescale);
+  return kOk;
+}
+
+int build_level(Context &c, int level) {
+  LevelPlan &lp = c.levels[level];
+  const int nm = c.lv_main[level], nt = c.lv_ter[level];
+  // beta = ceil((nm + max_ter) / alpha): the q limbs padded at the front to
+  // max_ter terminal limbs (ModSwitch.cu:181-185, :200-203)
+  int rc = build_modup(c, lp, nm, nt, c.alpha, c.max_ter - nt);
+  if (!rc) rc = build_moddown(c, nm, nt, nm, nt, c.alpha, 0, lp.moddown);
+  if (level == 0) return rc;  // no rescale below level 0
+  const int dm = c.lv_main[level - 1], dt = c.lv_ter[level - 1];
+  if (!rc) rc = build_moddown(c, nm, nt, dm, dt, c.alpha, 1, lp.rescale);
+  if (!rc) rc = build_moddown(c, nm, nt, dm, dt, c.alpha, 2, lp.moddown_rescale);
   return rc;
 }
 
@@ -305,6 +311,42 @@ bool np_less(int am, int at, int bm, int bt) {
 int global_prime(const Context &c, int num_main, int num_ter, int limb) {
   int num_q = num_main + num_ter;
   return limb < num_q ? c.max_ter - num_ter + limb : c.L + (limb - num_q);
+}
+
+// The level -1 ModSwitchHandler (ModSwitch.cu:175-232 with level -1: beta 1,
+// no padding, GetSSENumAux() = num_q auxiliary primes) and P' mod q for its
+// fold (Context.cpp:109-123). The short base must be the low end of the
+// bottom level (its terminal primes the last ones, its main primes the
+// first ones), with at most alpha primes.
+int build_short_base(Context &c, int num_main, int num_ter) {
+  const int num_q = num_main + num_ter;
+  if (num_main < 0 || num_ter < 0 || num_q < 1 || num_q > c.alpha ||
+      num_main > c.lv_main[0] || num_ter > c.lv_ter[0]) {
+    set_error("short base must be 1..alpha primes within level 0");
+    return kErrInvalidArg;
+  }
+  if (c.short_naux > 0) {
+    if (c.short_base.nm == num_main && c.short_base.nt == num_ter) return kOk;
+    set_error("a different short base is already set up");
+    return kErrInvalidArg;
+  }
+  LevelPlan lp;
+  int rc = build_modup(c, lp, num_main, num_ter, num_q, 0);
+  if (!rc) rc = build_moddown(c, num_main, num_ter, num_main, num_ter, num_q, 0, lp.moddown);
+  if (rc) return rc;
+  std::vector<uint32_t> pp(c.num_primes, 0);
+  for (int t = 0; t < num_q; t++) {
+    const int p = lp.q_prime[t];
+    const uint32_t q = c.primes[p];
+    uint32_t v = 1;
+    for (int j = 0; j < num_q; j++) v = h_mulmod(v, c.primes[c.L + j] % q, q);
+    pp[p] = h_to_mont(v, q);
+  }
+  rc = upload(c, &c.d_p_prod_short, pp);
+  if (rc) return rc;
+  c.short_base = std::move(lp);
+  c.short_naux = num_q;
+  return kOk;
 }
 
 int context_create(int log_n, const uint32_t *main_primes, int n_main,

@@ -26,8 +26,12 @@ struct EwOperand {
 
 struct KeyAccumArgs {
   const uint32_t *modup;  // beta blocks of num_total limbs
-  const uint32_t *key_b[kMaxDnum];  // full-size key limbs, global prime index
+  const uint32_t *key_b[kMaxDnum];  // key blocks (limb layout: key_q0, key_aux0)
   const uint32_t *key_a[kMaxDnum];
+  // key block limb of the first q limb and of the first aux limb (full-size
+  // blocks: the global primes max_ter - nt and L; the short-base key: 0 and
+  // num_q)
+  int key_q0, key_aux0;
   int group[kMaxDnum];
   int ngroups;
   // ModUp source (NTT domain, num_q limbs): a group's own limbs [own0, own1)

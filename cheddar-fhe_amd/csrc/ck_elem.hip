@@ -334,7 +334,8 @@ __device__ __forceinline__ void key_accum_body(const EwNp &np,
   const size_t tot_stride = (size_t)np.num_total * 65536;
   uint32_t rb[4] = {0, 0, 0, 0}, ra[4] = {0, 0, 0, 0};
   const size_t my = (size_t)y * 65536 + k;
-  const size_t kp = (size_t)p * 65536 + k;
+  const int kl = y < np.num_q ? A.key_q0 + y : A.key_aux0 + (y - np.num_q);
+  const size_t kp = (size_t)kl * 65536 + k;
   // HMult form: Tensor (ElementWise.cu:504-546) of the two input ciphertexts
   // on this q limb
   uint32_t d0[4] = {0, 0, 0, 0}, d1[4] = {0, 0, 0, 0}, d2[4] = {0, 0, 0, 0};

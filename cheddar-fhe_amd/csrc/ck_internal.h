@@ -160,6 +160,14 @@ struct Context {
   const uint32_t *i8_tw = nullptr;
 
   std::vector<LevelPlan> levels;
+  // Key switching at level -1 (the ModSwitchHandler of level -1 with
+  // Parameter::GetSSENumAux auxiliary primes, ModSwitch.cu:175-232): the
+  // short base's q limbs form one ModUp group, the first short_naux
+  // auxiliary primes are its P', and d_p_prod_short holds P' mod q * R by
+  // global prime (Context.cpp:109-123). short_naux = 0: not set up.
+  LevelPlan short_base;
+  int short_naux = 0;
+  uint32_t *d_p_prod_short = nullptr;
 
   // BaseConv as its own step on the matrix cores instead of fused into the
   // forward column pass: 1 FP64 matrix cores everywhere, 2 int8 matrix
@@ -231,6 +239,20 @@ void set_error(const std::string &msg);
 const char *last_error();
 int check_hip(hipError_t e, const char *what);
 int global_prime(const Context &c, int num_main, int num_ter, int limb);
+// level -1: the short base of num_main + num_ter primes (the bottom level's
+// primes less the additional base) with as many auxiliary primes
+int build_short_base(Context &c, int num_main, int num_ter);
+// the key-switching plan and auxiliary prime count of a level (-1 = the
+// short base when built)
+inline bool has_switch_level(const Context &c, int level) {
+  return (level >= 0 && level <= c.max_level) || (level == -1 && c.short_naux > 0);
+}
+inline const LevelPlan &level_plan(const Context &c, int level) {
+  return level < 0 ? c.short_base : c.levels[level];
+}
+inline int level_naux(const Context &c, int level) {
+  return level < 0 ? c.short_naux : c.alpha;
+}
 
 // host number theory (independent restatement of PrimeUtils.h)
 uint32_t h_mulmod(uint32_t a, uint32_t b, uint32_t q);

@@ -641,7 +641,8 @@ __device__ __forceinline__ void key_rows_body(const EwNp &np, const PrimeTab &P,
   const size_t tot_stride = (size_t)np.num_total * kN;
   const size_t off = (size_t)r * 256 + t16 * 16;  // this thread's 16 outputs
   const size_t my = (size_t)y * kN + off;
-  const size_t kp = (size_t)p * kN + off;
+  const int kl = y < np.num_q ? A.key_q0 + y : A.key_aux0 + (y - np.num_q);
+  const size_t kp = (size_t)kl * kN + off;
   const bool tens = A.use_tensor && y < np.num_q;
   // R^-1 and P = (P R) R^-1 as plain residues (Montgomery reductions)
   const double rinv = (double)mont_reduce(1, qu, qinv), rinvq = rinv * qr;

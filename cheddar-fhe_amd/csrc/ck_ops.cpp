@@ -226,8 +226,8 @@ int op_intt(const Context &c, uint32_t *dst, const uint32_t *src, int nm,
 int op_modup(const Context &c, int level, uint32_t *dst, const uint32_t *src,
              uint32_t *v, Sink &s, bool copy_own, const uint32_t *prod2,
              bool cols_only) {
-  const LevelPlan &lp = c.levels[level];
-  const int tot = lp.num_q + c.alpha;
+  const LevelPlan &lp = level_plan(c, level);
+  const int tot = lp.num_q + level_naux(c, level);
   // 1. INTT of all q limbs with the per-group constant, centred output
   {
     std::vector<PassJob> jobs;
@@ -303,8 +303,8 @@ int op_modup(const Context &c, int level, uint32_t *dst, const uint32_t *src,
 }
 
 const ModDownPlan *moddown_plan(const Context &c, int level, int kind) {
-  if (level < 0 || level > c.max_level) return nullptr;
-  const LevelPlan &lp = c.levels[level];
+  if (!has_switch_level(c, level) || (level < 0 && kind != 0)) return nullptr;
+  const LevelPlan &lp = level_plan(c, level);
   const ModDownPlan *mp = kind == 0   ? &lp.moddown
                           : kind == 1 ? &lp.rescale
                                       : &lp.moddown_rescale;
@@ -413,8 +413,8 @@ void fill_groups(const LevelPlan &lp, const uint32_t *const *key_b,
 }  // namespace
 
 size_t mult_key_ws_limbs(const Context &c, int level) {
-  const LevelPlan &lp = c.levels[level];
-  return (size_t)lp.num_q + (size_t)lp.beta * (lp.num_q + c.alpha);
+  const LevelPlan &lp = level_plan(c, level);
+  return (size_t)lp.num_q + (size_t)lp.beta * (lp.num_q + level_naux(c, level));
 }
 
 // MultKeyNoModDown (Context.cpp:609-712): ModUp(x), key inner product,
@@ -427,7 +427,8 @@ int op_mult_key_no_moddown(const Context &c, int level, uint32_t *acc_b,
                            const uint32_t *const *key_b,
                            const uint32_t *const *key_a, uint32_t *ws,
                            Sink &s, const uint32_t *const *tensor) {
-  const LevelPlan &lp = c.levels[level];
+  const LevelPlan &lp = level_plan(c, level);
+  const int naux = level_naux(c, level);
   uint32_t *v = ws;
   uint32_t *modup = ws + (size_t)lp.num_q * kLimb;
   const bool fused = c.fuse_key_rows;
@@ -445,16 +446,27 @@ int op_mult_key_no_moddown(const Context &c, int level, uint32_t *acc_b,
     for (int i = 0; i < 4; i++) A.tensor[i] = tensor[i];
     A.use_tensor = 1;
   }
-  A.p_prod = c.d_p_prod;
+  // level -1: the key block holds the short base's q limbs then its P'
+  // limbs (the key's own NP, prime offset 0: Context.cpp:619-646); other
+  // levels: full-size blocks indexed by global prime
+  if (level < 0) {
+    A.key_q0 = 0;
+    A.key_aux0 = lp.num_q;
+    A.p_prod = c.d_p_prod_short;
+  } else {
+    A.key_q0 = c.max_ter - lp.nt;
+    A.key_aux0 = c.L;
+    A.p_prod = c.d_p_prod;
+  }
   A.out_b = acc_b;
   A.out_a = acc_a;
-  const EwNp np = make_np(c, lp.nm, lp.nt, c.alpha);
+  const EwNp np = make_np(c, lp.nm, lp.nt, naux);
   return fused ? s.key_rows(np, A) : s.key_accum(np, A);
 }
 
 size_t relin_ws_limbs(const Context &c, int level) {
-  const LevelPlan &lp = c.levels[level];
-  const size_t tot = (size_t)lp.num_q + c.alpha;
+  const LevelPlan &lp = level_plan(c, level);
+  const size_t tot = (size_t)lp.num_q + level_naux(c, level);
   // accumulators + max(ModUp workspace, ModDown workspace of two polys)
   return 2 * tot + std::max(mult_key_ws_limbs(c, level), 2 * (size_t)tot);
 }
@@ -506,8 +518,8 @@ int op_mult_key(const Context &c, int level, uint32_t *out_b, uint32_t *out_a,
                 const uint32_t *b, const uint32_t *a,
                 const uint32_t *const *key_b, const uint32_t *const *key_a,
                 uint32_t *ws, Sink &s) {
-  const LevelPlan &lp = c.levels[level];
-  const int tot = lp.num_q + c.alpha;
+  const LevelPlan &lp = level_plan(c, level);
+  const int tot = lp.num_q + level_naux(c, level);
   uint32_t *acc_b = ws;
   uint32_t *acc_a = ws + (size_t)tot * kLimb;
   uint32_t *rest = ws + 2 * (size_t)tot * kLimb;

@@ -70,6 +70,18 @@ int ck_context_destroy(ck_context *ctx);
 /* out[0..9] = log_n, N, max_main, max_ter, alpha, L, dnum, max_level,
  * num_primes, workspace bytes / 2^20 */
 int ck_context_info(const ck_context *ctx, int *out);
+/* Level -1 for sparse-secret encapsulation: the short base (Parameter's
+ * short_base_np_, src/core/Parameter.cu:139-147: level 0's primes less the
+ * additional base) with as many auxiliary primes (GetSSENumAux,
+ * :271-274), the level -1 ModSwitchHandler (src/core/ModSwitch.cu:175-232)
+ * and P' mod q (p_prod_dts_, src/core/Context.cpp:109-123). Afterwards
+ * ck_level_to_np, ck_modup_beta, ck_modup, ck_moddown (kind 0),
+ * ck_mult_key_no_moddown and ck_mult_key accept level -1 (Context::MultKey
+ * with the DtS handler, src/core/Context.cpp:567-607); its key blocks hold
+ * the short base's limbs followed by its auxiliary limbs (2 num_q limbs).
+ * num_main + num_ter in 1..alpha; a second call with the same base is a
+ * no-op. */
+int ck_context_set_short_base(ck_context *ctx, int num_main, int num_ter);
 /* Parameter::LevelToNP (src/core/Parameter.cu:287-296) */
 int ck_level_to_np(const ck_context *ctx, int level, int *num_main,
                    int *num_ter);

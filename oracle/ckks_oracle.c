@@ -994,6 +994,206 @@ int or_mult_key_no_moddown(const or_param *p, int level, const uint32_t *x,
   return 0;
 }
 
+/* Context::MultKey at level -1: sparse-secret encapsulation's dense ->
+ * sparse key switch (Context.cpp:567-607; AdjustLevelForMultKey turns level
+ * 0 into -1 when the key's aux count differs from alpha). The level -1
+ * ModSwitchHandler (ModSwitch.cu:175-232) has beta 1, no front padding and
+ * num_aux = num_q auxiliary primes, the first ones (Parameter::
+ * LevelToNP(-1, n) and GetPrimeVector, Parameter.cu:287-296, :318-325):
+ *   ModUp(a)  (ModSwitch.cu:429-509): a's q limbs as they are, the aux limbs
+ *             by BaseConv of INTT(a) x c1 and a plain forward NTT;
+ *   PAccum    (Context.cpp:609-646, prime offset 0): acc = ModUp(a) (*) key
+ *             over the key's own NP (q limbs, then aux limbs);
+ *   CAccum    (Context.cpp:700-711): acc_b[q] += p_prod_dts (*) b, with
+ *             p_prod_dts = prod aux mod q in Montgomery form (:109-123);
+ *   ModDown   (ModSwitch.cu:566-684) of acc_b and acc_a: (acc[q] -
+ *             NTT(BaseConv(INTT(acc[aux]) x c1))) (*) (prod aux)^-1 R.
+ * key_b / key_a: 2 num_q limbs each. Returns -1 for an invalid base. */
+int or_mult_key_short_base(const or_param *p, int num_main, int num_ter,
+                           const uint32_t *b, const uint32_t *a,
+                           const uint32_t *key_b, const uint32_t *key_a,
+                           uint32_t *out_b, uint32_t *out_a) {
+  const int n = p->n, nq = num_main + num_ter, tot = 2 * nq;
+  if (num_main < 0 || num_ter < 0 || nq < 1 || nq > p->alpha ||
+      num_main > p->max_main || num_ter > p->max_ter)
+    return -1;
+  const size_t qs = (size_t)nq * n;
+  int *lp = (int *)malloc(sizeof(int) * tot);
+  level_prime_list(p, num_main, num_ter, nq, lp);
+  const int *aux = lp + nq;
+  uint32_t *c1 = (uint32_t *)malloc(sizeof(uint32_t) * nq);
+  int32_t *table = (int32_t *)malloc(sizeof(int32_t) * (size_t)nq * nq);
+  int32_t *v = (int32_t *)malloc(sizeof(int32_t) * qs);
+  uint32_t *mu = (uint32_t *)malloc(sizeof(uint32_t) * 2 * qs);
+  uint32_t *acc = (uint32_t *)malloc(sizeof(uint32_t) * 4 * qs);
+  uint32_t *y = (uint32_t *)malloc(sizeof(uint32_t) * qs);
+  uint32_t *pp = (uint32_t *)malloc(sizeof(uint32_t) * nq);
+  uint32_t *inv_prod = (uint32_t *)malloc(sizeof(uint32_t) * nq);
+
+  /* ModUp: the one group (every q limb) to the aux primes */
+  or_bconv_tables(p, lp, nq, aux, nq, NULL, 0, c1, table);
+  or_intt_mult_const(p, lp, nq, a, c1, 1, v);
+  memcpy(mu, a, sizeof(uint32_t) * qs);
+  or_bconv(p, lp, nq, v, aux, nq, table, mu + qs);
+  for (int j = 0; j < nq; j++) or_ntt_limb(p, aux[j], mu + qs + (size_t)j * n, 0);
+
+  /* key inner product and the P' b fold */
+  for (int i = 0; i < nq; i++) {
+    uint32_t q = p->q[lp[i]], prod = 1;
+    for (int j = 0; j < nq; j++) prod = or_mulmod(prod, p->q[aux[j]] % q, q);
+    pp[i] = or_to_mont(prod, q);
+    inv_prod[i] = or_to_mont(or_invmod(prod, q), q);
+  }
+  uint32_t *acc_b = acc, *acc_a = acc + 2 * qs;
+  for (int t = 0; t < tot; t++) {
+    const uint32_t q = p->q[lp[t]];
+    const int32_t qinv = p->qinv[lp[t]];
+    for (int k = 0; k < n; k++) {
+      const size_t o = (size_t)t * n + k;
+      uint32_t rb = or_mult_mont(mu[o], key_b[o], q, qinv);
+      const uint32_t ra = or_mult_mont(mu[o], key_a[o], q, qinv);
+      if (t < nq) rb = add_mod(rb, or_mult_mont(pp[t], b[o], q, qinv), q);
+      acc_b[o] = rb;
+      acc_a[o] = ra;
+    }
+  }
+
+  /* ModDown of both accumulators: aux -> q */
+  or_bconv_tables(p, aux, nq, lp, nq, NULL, 0, c1, table);
+  uint32_t *outs[2] = {out_b, out_a};
+  const uint32_t *accs[2] = {acc_b, acc_a};
+  for (int w = 0; w < 2; w++) {
+    or_intt_mult_const(p, aux, nq, accs[w] + qs, c1, 1, v);
+    or_bconv(p, aux, nq, v, lp, nq, table, y);
+    for (int i = 0; i < nq; i++) {
+      const uint32_t q = p->q[lp[i]];
+      const int32_t qinv = p->qinv[lp[i]];
+      uint32_t *yl = y + (size_t)i * n;
+      or_ntt_limb(p, lp[i], yl, 0);
+      for (int k = 0; k < n; k++) {
+        const size_t o = (size_t)i * n + k;
+        outs[w][o] = or_mult_mont(sub_mod(accs[w][o], yl[k], q), inv_prod[i], q, qinv);
+      }
+    }
+  }
+  free(lp);
+  free(c1);
+  free(table);
+  free(v);
+  free(mu);
+  free(acc);
+  free(y);
+  free(pp);
+  free(inv_prod);
+  return 0;
+}
+
+/* BootContext::ModUpToMax (BootContext.cpp:186-268): the level -1
+ * ciphertext (short base (nm, nt), 1 or 2 primes) raised to every q prime
+ * of the top level.
+ *   sse: MultKey at level -1 with the dense -> sparse key first;
+ *   per polynomial: INTTAndMultConst with ((prod_{j!=i} q_j) N)^-1 (plain;
+ *     BootContext.cpp:93-107), ModUpToMax1/2 (ElementWise.cu:579-623: one
+ *     prime s0 mod t, two primes s0 q1 + s1 q0 brought into (-q0 q1 / 2,
+ *     q0 q1 / 2]), NTT with Montgomery conversion; with sse a's lift also
+ *     covers the alpha auxiliary primes;
+ *   sse: PMult by the sparse -> dense key's block 0, acc_b[q] += P (*) b
+ *     (CAccum with GetPProd), ModDown of both (the top level's handler).
+ * Keys: dts_* 2 (nm + nt) limbs (short base, then auxiliary); std_* the top
+ * level's q limbs then alpha auxiliary limbs. Output: the top level's q
+ * limbs. Returns -1 for an unsupported base. */
+int or_mod_up_to_max(const or_param *p, int nm, int nt, int sse,
+                     const uint32_t *b, const uint32_t *a,
+                     const uint32_t *dts_b, const uint32_t *dts_a,
+                     const uint32_t *std_b, const uint32_t *std_a,
+                     uint32_t *out_b, uint32_t *out_a) {
+  const int n = p->n, k = nm + nt, alpha = p->alpha, top = p->max_level;
+  if (k < 1 || k > 2) return -1;
+  const int tm = p->lv_main[top], tt = p->lv_ter[top], tq = tm + tt;
+  const size_t ks = (size_t)k * n, tqs = (size_t)tq * n;
+  uint32_t *w = (uint32_t *)malloc(sizeof(uint32_t) * 2 * ks);
+  if (sse) {
+    if (or_mult_key_short_base(p, nm, nt, b, a, dts_b, dts_a, w, w + ks)) {
+      free(w);
+      return -1;
+    }
+  } else {
+    memcpy(w, b, sizeof(uint32_t) * ks);
+    memcpy(w + ks, a, sizeof(uint32_t) * ks);
+  }
+  int base[2], *lp = (int *)malloc(sizeof(int) * (tq + alpha));
+  uint32_t c[2];
+  for (int i = 0; i < k; i++) base[i] = or_np_prime(p, nm, nt, i);
+  for (int i = 0; i < k; i++) {
+    const uint32_t q = p->q[base[i]];
+    uint32_t v = 1;
+    for (int j = 0; j < k; j++)
+      if (j != i) v = or_mulmod(v, p->q[base[j]] % q, q);
+    c[i] = or_invmod(or_mulmod(v, (uint32_t)n % q, q), q);
+  }
+  level_prime_list(p, tm, tt, alpha, lp);
+  int32_t *coef = (int32_t *)malloc(sizeof(int32_t) * ks);
+  uint32_t *lift = (uint32_t *)malloc(sizeof(uint32_t) * 2 * (tqs + (size_t)alpha * n));
+  uint32_t *lb = lift, *la = lift + tqs + (size_t)alpha * n;
+  const int64_t q0 = p->q[base[0]], q1 = k == 2 ? p->q[base[1]] : 1, qq = q0 * q1;
+  for (int poly = 0; poly < 2; poly++) {
+    const int nl = tq + ((sse && poly == 1) ? alpha : 0);
+    uint32_t *dst = poly == 0 ? lb : la;
+    or_intt_mult_const(p, base, k, w + poly * ks, c, 0, coef);
+    for (int t = 0; t < nl; t++) {
+      const int64_t m = p->q[lp[t]];
+      for (int x = 0; x < n; x++) {
+        int64_t v;
+        if (k == 2) {
+          v = (int64_t)(uint32_t)coef[x] * q1 + (int64_t)(uint32_t)coef[n + x] * q0;
+          if (v >= qq) v -= qq;
+          if (v > qq / 2) v -= qq;
+        } else {
+          v = (uint32_t)coef[x];
+        }
+        int64_t r = v % m;
+        if (r < 0) r += m;
+        dst[(size_t)t * n + x] = (uint32_t)r;
+      }
+      or_ntt_limb(p, lp[t], dst + (size_t)t * n, 1);
+    }
+  }
+  if (!sse) {
+    memcpy(out_b, lb, sizeof(uint32_t) * tqs);
+    memcpy(out_a, la, sizeof(uint32_t) * tqs);
+  } else {
+    const int tot = tq + alpha;
+    uint32_t *acc = (uint32_t *)malloc(sizeof(uint32_t) * 2 * (size_t)tot * n);
+    uint32_t *acc_b = acc, *acc_a = acc + (size_t)tot * n;
+    for (int t = 0; t < tot; t++) {
+      const uint32_t q = p->q[lp[t]];
+      const int32_t qinv = p->qinv[lp[t]];
+      for (int x = 0; x < n; x++) {
+        const size_t o = (size_t)t * n + x;
+        uint32_t rb = or_mult_mont(std_b[o], la[o], q, qinv);
+        acc_a[o] = or_mult_mont(std_a[o], la[o], q, qinv);
+        if (t < tq) rb = add_mod(rb, or_mult_mont(lb[o], p->p_prod_mont[lp[t]], q, qinv), q);
+        acc_b[o] = rb;
+      }
+    }
+    int rc = or_moddown(p, top, OR_MODDOWN, acc_b, out_b);
+    if (!rc) rc = or_moddown(p, top, OR_MODDOWN, acc_a, out_a);
+    free(acc);
+    if (rc) {
+      free(w);
+      free(lp);
+      free(coef);
+      free(lift);
+      return -1;
+    }
+  }
+  free(w);
+  free(lp);
+  free(coef);
+  free(lift);
+  return 0;
+}
+
 /* Context::HMult (Context.cpp:862-871) = Mult (Tensor, :316-330) then
  * RelinearizeRescale (:714-737) or Relinearize -> MultKey (:584-607). */
 int or_hmult(const or_param *p, int level, const uint32_t *b1,

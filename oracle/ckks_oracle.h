@@ -175,6 +175,23 @@ int or_mult_key_no_moddown(const or_param *p, int level, const uint32_t *x,
                            const uint32_t *add_b, const uint32_t *add_a,
                            const uint32_t *evk_b, const uint32_t *evk_a,
                            uint32_t *accum_b, uint32_t *accum_a);
+/* MultKey at level -1 (the dense -> sparse switch of sparse-secret
+ * encapsulation) on the short base (num_main, num_ter) with as many
+ * auxiliary primes; key_b / key_a hold 2 num_q limbs (short base, then
+ * auxiliary) */
+int or_mult_key_short_base(const or_param *p, int num_main, int num_ter,
+                           const uint32_t *b, const uint32_t *a,
+                           const uint32_t *key_b, const uint32_t *key_a,
+                           uint32_t *out_b, uint32_t *out_a);
+/* BootContext::ModUpToMax from the short base (nm, nt) (1 or 2 primes) to
+ * the top level's q limbs; with sse between the dense -> sparse switch
+ * (dts_*: 2 (nm + nt) limbs) and the sparse -> dense switch at the top
+ * level (std_*: the top level's q limbs then alpha auxiliary limbs) */
+int or_mod_up_to_max(const or_param *p, int nm, int nt, int sse,
+                     const uint32_t *b, const uint32_t *a,
+                     const uint32_t *dts_b, const uint32_t *dts_a,
+                     const uint32_t *std_b, const uint32_t *std_a,
+                     uint32_t *out_b, uint32_t *out_a);
 /* HMult = Tensor + RelinearizeRescale (rescale=1) or Relinearize (0).
  * Output limbs: num_q(level - 1) when rescaling, num_q(level) otherwise. */
 int or_hmult(const or_param *p, int level, const uint32_t *b1,

@@ -83,6 +83,10 @@ def _load():
         "or_hmult": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [_u32p] * 6 + [ctypes.c_int, _u32p, _u32p]),
         "or_hrot": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [_u32p] * 4 + [ctypes.c_int, _u32p, _u32p]),
         "or_hconj": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [_u32p] * 6),
+        "or_mult_key_short_base": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+                                   + [_u32p] * 6),
+        "or_mod_up_to_max": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int] + [_u32p] * 8),
         "or_hoisted_baby_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [_u32p] * 4 +
                                  [ctypes.c_int, _u32p, _u32p]),
     }
@@ -365,6 +369,41 @@ class Param:
         rc = lib.or_hmult(self._h, level, *[_p(a) for a in ins], int(rescale), _p(ob), _p(oa))
         if rc != 0:
             raise ValueError("or_hmult failed")
+        return ob, oa
+
+    def mult_key_short_base(self, num_main, num_ter, ct, key):
+        """MultKey at level -1 (sparse-secret encapsulation's dense -> sparse
+        switch, src/core/Context.cpp:567-607 with the level -1
+        ModSwitchHandler, src/core/ModSwitch.cu:175-232): ct over the short
+        base's num_q limbs, key = (b, a) over 2 num_q limbs"""
+        nq = num_main + num_ter
+        ob = np.empty((nq, self.n), dtype=np.uint32)
+        oa = np.empty((nq, self.n), dtype=np.uint32)
+        ins = [np.ascontiguousarray(v, dtype=np.uint32) for v in (ct[0], ct[1], key[0], key[1])]
+        rc = lib.or_mult_key_short_base(self._h, num_main, num_ter, *[_p(a) for a in ins],
+                                        _p(ob), _p(oa))
+        if rc != 0:
+            raise ValueError("or_mult_key_short_base failed")
+        return ob, oa
+
+    def mod_up_to_max_ct(self, num_main, num_ter, ct, dts=None, std=None):
+        """BootContext::ModUpToMax (src/extension/BootContext.cpp:186-268) of
+        a ciphertext over the short base (1 or 2 primes) to the top level's
+        q limbs; with dts / std keys (sparse-secret encapsulation) between
+        the dense -> sparse switch at level -1 and the sparse -> dense switch
+        at the top level. dts: 2 num_q limbs each; std: the top level's q
+        limbs then alpha auxiliary limbs each."""
+        sse = dts is not None
+        tq = self.num_q(self.max_level)
+        ob = np.empty((tq, self.n), dtype=np.uint32)
+        oa = np.empty((tq, self.n), dtype=np.uint32)
+        ins = [np.ascontiguousarray(v, dtype=np.uint32) for v in ct]
+        keys = [np.ascontiguousarray(v, dtype=np.uint32) if sse else None
+                for v in ((dts or (None, None)) + (std or (None, None)))]
+        rc = lib.or_mod_up_to_max(self._h, num_main, num_ter, int(sse), _p(ins[0]), _p(ins[1]),
+                                  *[_p(k) for k in keys], _p(ob), _p(oa))
+        if rc != 0:
+            raise ValueError("or_mod_up_to_max failed")
         return ob, oa
 
     def hconj(self, level, ct, ck):

@@ -29,6 +29,7 @@
 #include "cheddar/UserInterface.h"
 #include "cheddar/extension/BootContext.h"
 #include "cheddar/extension/BootGraph.h"
+#include "ckks_oracle.h"
 
 using namespace cheddar;
 using word = uint32_t;
@@ -117,6 +118,61 @@ int main(int argc, char **argv) {
     ctx->encoder_.Decode(m, pt);
     return m;
   };
+
+  try {
+    // ModUpToMax (BootContext.cpp:186-268) word for word against the oracle:
+    // a uniform level -1 ciphertext, with the facade's own DtS / StD keys
+    // when encapsulating
+    std::vector<int> cfg;
+    for (auto &c : P.level_config_) {
+      cfg.push_back(c.first);
+      cfg.push_back(c.second);
+    }
+    or_set_num_threads(8);
+    or_param *op = or_param_create(P.log_degree_, P.main_primes_.data(), (int)P.main_primes_.size(),
+                                   P.ter_primes_.data(), (int)P.ter_primes_.size(),
+                                   P.aux_primes_.data(), (int)P.aux_primes_.size(), cfg.data(),
+                                   (int)P.level_config_.size());
+    if (!op) throw Error("oracle rejected the parameters");
+    const NPInfo base = P.LevelToNP(-1);
+    const size_t bw = (size_t)base.GetNumQ() * P.degree_;
+    std::vector<word> hb(bw), ha(bw);
+    or_fill_uniform(op, base.num_main_, base.num_ter_, 0, 0x4348454444415205ull, hb.data());
+    or_fill_uniform(op, base.num_main_, base.num_ter_, 0, 0x4348454444415206ull, ha.data());
+    Ct ct, out;
+    ct.ModifyNP(base);
+    ct.bx_.CopyFromHost(hb);
+    ct.ax_.CopyFromHost(ha);
+    ctx->ModUpToMax(out, ct, keys);
+    const std::vector<word> gb = out.bx_.CopyToHost(), ga = out.ax_.CopyToHost();
+    const size_t tw = (size_t)P.LevelToNP(P.max_level_).GetNumQ() * P.degree_;
+    std::vector<word> wb(tw), wa(tw);
+    int rc;
+    if (P.IsUsingSparseSecretEncapsulation()) {
+      const EvaluationKey<word> &dts = keys.GetDenseToSparseKey();
+      const EvaluationKey<word> &sd = keys.GetSparseToDenseKey();
+      const std::vector<word> db = dts.bx_[0].CopyToHost(), da = dts.ax_[0].CopyToHost();
+      const std::vector<word> sb = sd.bx_[0].CopyToHost(), sa = sd.ax_[0].CopyToHost();
+      rc = or_mod_up_to_max(op, base.num_main_, base.num_ter_, 1, hb.data(), ha.data(),
+                            db.data(), da.data(), sb.data(), sa.data(), wb.data(), wa.data());
+    } else {
+      rc = or_mod_up_to_max(op, base.num_main_, base.num_ter_, 0, hb.data(), ha.data(), nullptr,
+                            nullptr, nullptr, nullptr, wb.data(), wa.data());
+    }
+    or_param_destroy(op);
+    size_t diff = 0;
+    for (size_t i = 0; i < tw; i++) diff += (gb.size() == tw && gb[i] != wb[i]) + (ga.size() == tw && ga[i] != wa[i]);
+    const bool same = rc == 0 && gb.size() == tw && ga.size() == tw && diff == 0;
+    std::printf("%s ModUpToMax%s vs the oracle, word for word: %zu of %zu words differ\n",
+                same ? "PASS" : "FAIL",
+                P.IsUsingSparseSecretEncapsulation() ? " (dense -> sparse -> dense)" : "", diff,
+                2 * tw);
+    std::fflush(stdout);
+    if (!same) g_fail++;
+  } catch (const std::exception &e) {
+    std::printf("FAIL ModUpToMax: %s\n", e.what());
+    g_fail++;
+  }
 
   try {
     // EvalMod: t = (I + r) / K -> ~ sin(2 pi r) / (2 pi)
@@ -225,6 +281,13 @@ int main(int argc, char **argv) {
     (void)hipDeviceSynchronize();
     std::printf("boot (min-KS): %.2f ms per bootstrap\n", 1e3 * Seconds(t));
     Report("Boot min-KS (full slots)", m, decrypt(out), 1e-3);
+    if (P.IsUsingSparseSecretEncapsulation()) {
+      // the non-default dense -> sparse switch with exact CRT lifts
+      ctx->SetSseExactLift(true);
+      ctx->Boot(out, ct, keys);
+      ctx->SetSseExactLift(false);
+      Report("Boot, exact-lift dense -> sparse switch (full slots)", m, decrypt(out), 1e-3);
+    }
   } catch (const std::exception &e) {
     std::printf("FAIL Boot: %s\n", e.what());
     g_fail++;

@@ -170,3 +170,131 @@ def test_hoisted_then_moddown_equals_hrot(oracle_rns24, level, rot):
     want = P.hrot(level, ct, key, rot)
     assert np.array_equal(got[0], want[0])
     assert np.array_equal(got[1], want[1])
+
+
+# ---- level -1: the dense -> sparse switch of sparse-secret encapsulation ----
+
+def test_short_base_mult_key_equals_level_switch_where_they_coincide(oracle_rns24):
+    """rns24's level 7 has 8 main primes = alpha and no terminal primes, so
+    its ModSwitchHandler is a single unpadded group raised to all 8
+    auxiliary primes: exactly the level -1 handler of a short base of those
+    8 primes (ModSwitch.cu:175-232). MultKey there (Context.cpp:584-607)
+    and the short-base MultKey agree word for word."""
+    from gpu_util import full_key
+    P = oracle_rns24
+    level = 7
+    nm, nt = P.level_np(level)
+    assert (nm, nt) == (8, 0) and P.alpha == 8 and P.modup_beta(level) == 1
+    pidx = P.level_primes(level)
+    ct = [P.uniform(pidx, SEED_BASE + 740 + i) for i in range(2)]
+    kb, ka = full_key(P, SEED_BASE + 750)
+    acc_b, acc_a = P.mult_key_no_moddown(level, ct[1], kb, ka, add_b=ct[0])
+    want = [P.moddown(v, level, O.OR_MODDOWN) for v in (acc_b, acc_a)]
+    rows = pidx + [P.L + j for j in range(8)]
+    got = P.mult_key_short_base(nm, nt, ct, (kb[0][rows], ka[0][rows]))
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+
+
+def dts_key(P, dense, sparse, nm, nt, err=3):
+    """the dense -> sparse key over the short base and its auxiliary primes
+    (UserInterface.cu PrepareEvk for kDenseToSparseKeyIndex): b = -a s' + e
+    + P' s on the short base's limbs, -a s' + e on the auxiliary limbs"""
+    qp = P.np_primes(nm, nt)
+    k = len(qp)
+    pidx = qp + [P.L + j for j in range(k)]
+    a = dense.uniform(pidx)
+    e = dense.ntt_small(dense.small(err), pidx)
+    b = P.sub(e, P.mult(a, sparse.s[pidx], pidx), pidx)
+    pp = []
+    for p in qp:
+        q = P.prime(p)
+        v = 1
+        for j in range(k):
+            v = v * P.prime(P.L + j) % q
+        pp.append(v * (1 << 32) % q)
+    b[:k] = P.add(b[:k], P.mult_const(dense.s[qp], np.array(pp, dtype=np.uint32), qp), qp)
+    return b, a
+
+
+@pytest.mark.parametrize("name", ["bootparam_40.json", "bootparam_30.json"])
+def test_dts_mult_key_switches_dense_to_sparse(name):
+    """BootContext::ModUpToMax's dense -> sparse switch (BootContext.cpp:
+    205-210 -> Context::MultKey at level -1): a level-0 encryption under the
+    dense secret decrypts under the sparse secret (Hamming weight 32) after
+    the switch, up to the key-switching error. bootparam_40's short base is
+    two terminal primes, bootparam_30's two main primes."""
+    from conftest import load_params
+    P = O.Param.from_json(load_params(name))
+    nm, nt = P.level_np(0)
+    dense = Keys(P, SEED_BASE + 760)
+    sparse = Keys(P, SEED_BASE + 761, hamming=32)
+    key = dts_key(P, dense, sparse, nm, nt)
+    rng = np.random.default_rng(7)
+    m = rng.integers(-2 ** 30, 2 ** 30, size=P.n)  # |m| < Q / 2^9
+    ct = dense.encrypt(m, 0)
+    out = P.mult_key_short_base(nm, nt, ct, key)
+    ks = [0, 1, P.n - 1] + [int(v) for v in rng.integers(0, P.n, size=5)]
+    for k, v in zip(ks, sparse.decrypt(out, 0, ks)):
+        assert abs(v - m[k]) < 2 ** 16, (k, v, m[k])
+    # the switched ciphertext does not decrypt under the dense secret
+    # (bootparam_30's short base is two ~2^20 primes: Q ~ 2^40)
+    Q = 1
+    for p in P.np_primes(nm, nt):
+        Q *= P.prime(p)
+    bad = dense.decrypt(out, 0, ks[:3])
+    assert any(abs(v - m[k]) > Q >> 12 for k, v in zip(ks[:3], bad))
+
+
+def std_key(P, dense, sparse, err=3):
+    """the sparse -> dense key at the top level, one block over the top
+    level's q limbs and the alpha auxiliary limbs: b = -a s + e + P s' on
+    the q limbs (the whole of Q as one digit: the raised a is a lift of a
+    short-base value, so a e / P stays small)"""
+    tq = P.level_primes(P.max_level)
+    pidx = tq + [P.L + j for j in range(P.alpha)]
+    a = dense.uniform(pidx)
+    e = dense.ntt_small(dense.small(err), pidx)
+    b = P.sub(e, P.mult(a, dense.s[pidx], pidx), pidx)
+    pp = np.array([P.p_prod_mont(p) for p in tq], dtype=np.uint32)
+    nq = len(tq)
+    b[:nq] = P.add(b[:nq], P.mult_const(sparse.s[tq], pp, tq), tq)
+    return b, a
+
+
+@pytest.mark.parametrize("name,sse", [("bootparam_40.json", False),
+                                      ("bootparam_40.json", True),
+                                      ("bootparam_30.json", True)])
+def test_mod_up_to_max_lifts_by_multiples_of_q0(name, sse):
+    """BootContext::ModUpToMax (BootContext.cpp:186-268): the raised
+    ciphertext decrypts at the top level to m + e + Q0 I with |I| at most
+    the sparse Hamming weight + 2 (the lift of b + a s' from the short base
+    Q0), under the sparse secret without encapsulation and under the dense
+    secret after the dense -> sparse -> dense switches (then e also holds
+    the two key-switching errors)."""
+    from conftest import load_params
+    P = O.Param.from_json(load_params(name))
+    nm, nt = P.level_np(0)
+    dense = Keys(P, SEED_BASE + 770)
+    sparse = Keys(P, SEED_BASE + 771, hamming=32)
+    owner = dense if sse else sparse
+    rng = np.random.default_rng(11)
+    m = rng.integers(-2 ** 20, 2 ** 20, size=P.n)
+    ct = owner.encrypt(m, 0)
+    if sse:
+        out = P.mod_up_to_max_ct(nm, nt, ct, dts_key(P, dense, sparse, nm, nt),
+                                 std_key(P, dense, sparse))
+    else:
+        out = P.mod_up_to_max_ct(nm, nt, ct)
+    Q0 = 1
+    for p in P.np_primes(nm, nt):
+        Q0 *= P.prime(p)
+    ks = [0, 1, P.n - 1] + [int(v) for v in rng.integers(0, P.n, size=13)]
+    lifts = []
+    for k, v in zip(ks, owner.decrypt(out, P.max_level, ks)):
+        d = v - int(m[k])
+        r = (d + Q0 // 2) % Q0 - Q0 // 2
+        I = (d - r) // Q0
+        assert abs(r) < (2 ** 20 if sse else 2 ** 6), (k, r)
+        assert abs(I) <= 34, (k, I)
+        lifts.append(I)
+    assert any(I != 0 for I in lifts)
