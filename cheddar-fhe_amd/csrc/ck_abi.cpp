@@ -766,6 +766,9 @@ static int build_tuned(const Context &c, Build &build, Plan **out) {
   return kOk;
 }
 
+// true when one of a[0..na) is one of b[0..nb) (defined below)
+extern "C++" bool aliases(const void *const *a, int na, const void *const *b, int nb);
+
 int ck_plan_hmult(ck_context *ctx, int n, const int *levels, int rescale,
                   uint32_t *const *out_b, uint32_t *const *out_a,
                   const uint32_t *const *b1, const uint32_t *const *a1,
@@ -780,6 +783,15 @@ int ck_plan_hmult(ck_context *ctx, int n, const int *levels, int rescale,
   CK_REQUIRE(valid_ptrs(out_b, n) && valid_ptrs(out_a, n) && valid_ptrs(b1, n) &&
                  valid_ptrs(a1, n) && valid_ptrs(b2, n) && valid_ptrs(a2, n),
              "null pointer");
+  if (c.plan_autotune) {
+    // the timed candidate runs write the outputs before the caller's first
+    // run: they must not overwrite the inputs
+    const uint32_t *const *ins[4] = {b1, a1, b2, a2};
+    for (const uint32_t *const *in : ins)
+      CK_REQUIRE(!aliases((const void *const *)out_b, n, (const void *const *)in, n) &&
+                     !aliases((const void *const *)out_a, n, (const void *const *)in, n),
+                 "an autotuned plan's outputs must not alias its inputs");
+  }
   size_t total = 0;
   std::vector<size_t> off(n);
   for (int i = 0; i < n; i++) {
@@ -1060,6 +1072,15 @@ int ck_plan_hrot_hoisted(ck_context *ctx, int n, const int *levels,
   CK_REQUIRE(valid_ptrs(b, n) && valid_ptrs(a, n), "null input pointer");
   CK_REQUIRE(valid_ptrs(out_b, n * n_rot) && valid_ptrs(out_a, n * n_rot),
              "null output pointer");
+  if (c.plan_autotune) {
+    // the timed candidate runs write the outputs before the caller's first
+    // run: they must not overwrite the inputs
+    const uint32_t *const *ins[2] = {b, a};
+    for (const uint32_t *const *in : ins)
+      CK_REQUIRE(!aliases((const void *const *)out_b, n * n_rot, (const void *const *)in, n) &&
+                     !aliases((const void *const *)out_a, n * n_rot, (const void *const *)in, n),
+                 "an autotuned plan's outputs must not alias its inputs");
+  }
   size_t total = 0;
   std::vector<size_t> off(n);
   for (int i = 0; i < n; i++) {

@@ -141,3 +141,19 @@ def test_plan_graph_replays(ck, ctx24, oracle_rns24, key24):
         for i in range(len(levels)):
             for k in range(2):
                 assert torch.equal(outs[i][k], eager[i][k]), (rep, i, k)
+
+
+def test_autotuned_plans_reject_aliasing(ck, rns24, oracle_rns24, key24):
+    """the timed candidate runs of an autotuned plan write its outputs, so
+    outputs that alias inputs are refused at plan creation"""
+    P = oracle_rns24
+    ctx = ck.Context.from_json(rns24)
+    ctx.set_plan_autotune(True)
+    ct = [ck.to_device(uniform(P, P.level_primes(9), SEED_BASE + 3000 + 900 + j))
+          for j in range(2)]
+    other = [ck.to_device(uniform(P, P.level_primes(9), SEED_BASE + 3000 + 910 + j))
+             for j in range(2)]
+    with pytest.raises(ck.CheddarError):
+        ck.Plan.hmult(ctx, [9], [[ct[0], ct[1]]], [ct], [other], dev_key(*key24))
+    with pytest.raises(ck.CheddarError):
+        ck.Plan.hrot_hoisted(ctx, [9], [ct], [dev_key(*key24)], [5], [[[ct[0], other[1]]]])
