@@ -32,12 +32,6 @@ struct KeyAccumArgs {
   // blocks: the global primes max_ter - nt and L; the short-base key: 0 and
   // num_q)
   int key_q0, key_aux0;
-  // limbs whose accumulators feed a ModDown's INTT: the fused pass (key_rows)
-  // applies the inverse row stages to them and stores the lazy int32 row
-  // output instead of the NTT-domain value (bit y of inv_rows[y / 32]);
-  // tw_inv: the inverse twiddle table
-  uint32_t inv_rows[4];
-  const uint32_t *tw_inv;
   int group[kMaxDnum];
   int ngroups;
   // ModUp source (NTT domain, num_q limbs): a group's own limbs [own0, own1)

@@ -45,8 +45,6 @@ enum NttFlags : int {
   kInttLoadProduct = 16,    // inverse: input = src (*) src2 (Montgomery)
   kNttBconvInput = 32,      // forward: input = BaseConv of bsrc (see below)
   kNttColsOnly = 64,        // forward: column stages only (rows fused later)
-  kInttColsOnly = 128,      // inverse: row stages already applied (key_rows);
-                            // the column pass reads src
 };
 
 // Per-limb fields of a kernel-argument plan, kept together in one record
@@ -280,9 +278,7 @@ void launch_intt(const Context &ctx, const NttLaunch &L, hipStream_t st);
 // count among the jobs (selects the column kernel's instantiation)
 void launch_ntt_fwd_table(const Context &ctx, const LimbJob *jobs, int n,
                           int flags_any, int max_bs, hipStream_t st);
-// flags_all = AND of the jobs' flags: the row pass is skipped when every
-// job has kInttColsOnly (jobs with it return early from a row pass that runs)
 void launch_intt_table(const Context &ctx, const LimbJob *jobs, int n,
-                       int flags_any, int flags_all, hipStream_t st);
+                       int flags_any, hipStream_t st);
 
 }  // namespace ck

@@ -481,7 +481,6 @@ __global__ __launch_bounds__(kRowThreads, 5) void intt_rows(
   __shared__ double ldsd[kRowTile * 256];
   const int y = blockIdx.y;
   const LimbJob J = desc.job(y);
-  if (J.flags & kInttColsOnly) return;  // row stages already applied
   const double q = P.qd[J.prime], qr = P.qr[J.prime];
   const ModF m{q, qr};
   const uint32_t *__restrict__ w = tw + (size_t)J.prime * kN;
@@ -577,10 +576,8 @@ __global__ __launch_bounds__(kColThreads, 5) void intt_cols(
   // rows rb * 16 + j of column c0 + cc
   double x[16];
   {
-    // (the row stages ran in key_rows when kInttColsOnly: their output is
-    // the source)
-    const uint32_t *in = (J.flags & kInttColsOnly) ? J.src : J.dst;
-    const int32_t *__restrict__ g = reinterpret_cast<const int32_t *>(in) + c0 + cc;
+    const int32_t *__restrict__ g =
+        reinterpret_cast<const int32_t *>(J.dst) + c0 + cc;
 #pragma unroll
     for (int j = 0; j < 16; j++) x[j] = (double)g[(rb * 16 + j) * 256];
   }
@@ -628,44 +625,6 @@ __global__ __launch_bounds__(kColThreads, 5) void intt_cols(
 // exactly the key_accum kernel's sum of Montgomery products; here every
 // product is an exact FP64 modular product of lazy values and one product by
 // R^-1 ends the sum.
-// The inverse row stages of intt_rows on one 16-row tile whose thread
-// (rl, t16) holds the 16 canonical values 16 t16 + j of row r; the lazy
-// int32 output goes to the tile's place in `tile_out`, as intt_rows writes
-// it. Each phase loads its twiddles (psi_inv_rev of the prime, `wi`) just
-// before use, which keeps this epilogue within the kernel's registers.
-__device__ __forceinline__ void inv_rows_store(const uint32_t (&c)[16],
-                                               const uint32_t *__restrict__ wi, int r,
-                                               double q, double qr, const ModF &m,
-                                               double *ldsd, int rl, int t16,
-                                               uint32_t *tile_out) {
-  uint32_t t[15];
-#pragma unroll
-  for (int u = 0; u < 4; u++)
-#pragma unroll
-    for (int blk = 0; blk < (8 >> u); blk++)
-      t[16 - (16 >> u) + blk] = wi[(32768 >> u) + (r << (7 - u)) + (t16 << (3 - u)) + blk];
-  double x[16];
-#pragma unroll
-  for (int j = 0; j < 16; j++) x[j] = (double)c[j];
-  gs16(x, t, q, qr);
-  __syncthreads();  // the previous exchange may still be read
-#pragma unroll
-  for (int j = 0; j < 16; j++) ldsd[rlds(rl, 16 * t16 + j)] = x[j];
-  __syncthreads();
-#pragma unroll
-  for (int u = 0; u < 4; u++)
-#pragma unroll
-    for (int blk = 0; blk < (8 >> u); blk++)
-      t[16 - (16 >> u) + blk] = wi[(2048 >> u) + (r << (3 - u)) + blk];
-#pragma unroll
-  for (int k = 0; k < 16; k++) x[k] = ldsd[rlds(rl, t16 + 16 * k)];
-  gs16(x, t, q, qr);
-  int32_t o[16];
-#pragma unroll
-  for (int k = 0; k < 16; k++) o[k] = lazy_i32(x[k], m);
-  rows_store_p1(tile_out, rl, t16, o);
-}
-
 template <class Args>
 __device__ __forceinline__ void key_rows_body(const EwNp &np, const PrimeTab &P,
                                               Args &A, const uint32_t *tw, int y,
@@ -831,36 +790,18 @@ __device__ __forceinline__ void key_rows_body(const EwNp &np, const PrimeTab &P,
     }
   }
 
-  // canonical accumulators, one polynomial at a time (the scheduling
-  // barriers keep the two conversions from being interleaved, whose
-  // temporaries together exceed the registers)
-  uint32_t cb[16], ca[16];
-#pragma unroll
-  for (int j = 0; j < 16; j++) cb[j] = canon_small(mulmod_f(sb[j], rinv, rinvq, q), q);
-  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int j = 0; j < 16; j++) ca[j] = canon_small(mulmod_f(sa[j], rinv, rinvq, q), q);
-  __builtin_amdgcn_sched_barrier(0);
-  const uint32_t inv_word = y < 32   ? A.inv_rows[0]
-                            : y < 64 ? A.inv_rows[1]
-                            : y < 96 ? A.inv_rows[2]
-                                     : A.inv_rows[3];
-  if ((inv_word >> (y & 31)) & 1) {
-    // this limb feeds a ModDown: the INTT's row stages (intt_rows) on the
-    // canonical accumulators, whose 16 values per thread are exactly the
-    // elements 16 t16 + j that intt_rows' phase 1 holds
-    const uint32_t *__restrict__ wi = A.tw_inv + (size_t)p * kN;
-    const ModF m{q, qr};
-    inv_rows_store(cb, wi, r, q, qr, m, ldsd, rl, t16, A.out_b + (size_t)y * kN + tile);
-    inv_rows_store(ca, wi, r, q, qr, m, ldsd, rl, t16, A.out_a + (size_t)y * kN + tile);
-    return;
-  }
   uint4 *ob = reinterpret_cast<uint4 *>(A.out_b + my);
   uint4 *oa = reinterpret_cast<uint4 *>(A.out_a + my);
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    ob[i] = make_uint4(cb[4 * i], cb[4 * i + 1], cb[4 * i + 2], cb[4 * i + 3]);
-    oa[i] = make_uint4(ca[4 * i], ca[4 * i + 1], ca[4 * i + 2], ca[4 * i + 3]);
+    uint32_t vb[4], va[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      vb[e] = canon_small(mulmod_f(sb[4 * i + e], rinv, rinvq, q), q);
+      va[e] = canon_small(mulmod_f(sa[4 * i + e], rinv, rinvq, q), q);
+    }
+    ob[i] = make_uint4(vb[0], vb[1], vb[2], vb[3]);
+    oa[i] = make_uint4(va[0], va[1], va[2], va[3]);
   }
 }
 
@@ -913,11 +854,10 @@ void fwd_passes(const Context &ctx, const D &d, int n, int flags, int max_bs,
 }
 
 template <class D>
-void inv_passes(const Context &ctx, const D &d, int n, int flags, int flags_all,
+void inv_passes(const Context &ctx, const D &d, int n, int flags,
                 hipStream_t st) {
-  if (!(flags_all & kInttColsOnly))
-    launch(ctx, inv_rows_name(flags), n, intt_rows<D>, dim3(kRowTiles, n),
-           dim3(kRowThreads), st, d, ctx.ptab, (const uint32_t *)ctx.d_tw_inv);
+  launch(ctx, inv_rows_name(flags), n, intt_rows<D>, dim3(kRowTiles, n),
+         dim3(kRowThreads), st, d, ctx.ptab, (const uint32_t *)ctx.d_tw_inv);
   launch(ctx, "intt_cols", n, intt_cols<D>, dim3(kColTiles * n),
          dim3(kColThreads), st, d, ctx.ptab, (const uint32_t *)ctx.d_tw_inv);
 }
@@ -937,7 +877,7 @@ void launch_ntt_fwd(const Context &ctx, const NttLaunch &L, hipStream_t st) {
 // place on dst with the post constant c1 (plain form) and optional centring.
 void launch_intt(const Context &ctx, const NttLaunch &L, hipStream_t st) {
   if (L.n <= 0) return;
-  inv_passes(ctx, JobByValue{L}, L.n, L.flags, L.flags, st);
+  inv_passes(ctx, JobByValue{L}, L.n, L.flags, st);
 }
 
 void launch_ntt_fwd_table(const Context &ctx, const LimbJob *jobs, int n,
@@ -961,9 +901,9 @@ void launch_key_rows_table(const Context &ctx, const KeyAccumItem *items, const 
 }
 
 void launch_intt_table(const Context &ctx, const LimbJob *jobs, int n,
-                       int flags_any, int flags_all, hipStream_t st) {
+                       int flags_any, hipStream_t st) {
   if (n <= 0) return;
-  inv_passes(ctx, JobTable{jobs}, n, flags_any, flags_all, st);
+  inv_passes(ctx, JobTable{jobs}, n, flags_any, st);
 }
 
 }  // namespace ck

@@ -315,7 +315,7 @@ const ModDownPlan *moddown_plan(const Context &c, int level, int kind) {
 // a workspace of npoly * src_len limbs.
 int op_moddown(const Context &c, int level, int kind, int npoly,
                uint32_t *const *dst, const uint32_t *const *src, uint32_t *v,
-               Sink &s, bool rows_done) {
+               Sink &s) {
   const ModDownPlan *mp = moddown_plan(c, level, kind);
   if (!mp) {
     set_error("no ModDown plan for this level and kind");
@@ -335,7 +335,7 @@ int op_moddown(const Context &c, int level, int kind, int npoly,
         j.c1 = mp->c1[i];
         jobs.push_back(j);
       }
-    int rc = s.pass(false, kInttCentred | (rows_done ? kInttColsOnly : 0), jobs);
+    int rc = s.pass(false, kInttCentred, jobs);
     if (rc) return rc;
   }
   // 2. BaseConv (matrix cores, or fused into the forward NTT), 3. epilogue
@@ -426,8 +426,7 @@ int op_mult_key_no_moddown(const Context &c, int level, uint32_t *acc_b,
                            const uint32_t *add_b, const uint32_t *add_a,
                            const uint32_t *const *key_b,
                            const uint32_t *const *key_a, uint32_t *ws,
-                           Sink &s, const uint32_t *const *tensor,
-                           const ModDownPlan *fold_inv_rows) {
+                           Sink &s, const uint32_t *const *tensor) {
   const LevelPlan &lp = level_plan(c, level);
   const int naux = level_naux(c, level);
   uint32_t *v = ws;
@@ -461,21 +460,8 @@ int op_mult_key_no_moddown(const Context &c, int level, uint32_t *acc_b,
   }
   A.out_b = acc_b;
   A.out_a = acc_a;
-  if (fused && fold_inv_rows) {
-    for (int t : fold_inv_rows->src_limb) A.inv_rows[t >> 5] |= 1u << (t & 31);
-    A.tw_inv = c.d_tw_inv;
-  }
   const EwNp np = make_np(c, lp.nm, lp.nt, naux);
   return fused ? s.key_rows(np, A) : s.key_accum(np, A);
-}
-
-const ModDownPlan *fold_plan(const Context &c, int level, int kind) {
-  if (!c.fuse_key_rows) return nullptr;
-  const ModDownPlan *mp = moddown_plan(c, level, kind);
-  if (!mp) return nullptr;
-  for (int t : mp->src_limb)
-    if (t < 0 || t >= 128) return nullptr;
-  return mp;
 }
 
 size_t relin_ws_limbs(const Context &c, int level) {
@@ -497,13 +483,12 @@ int op_relinearize(const Context &c, int level, uint32_t *out_b,
   uint32_t *acc_b = ws;
   uint32_t *acc_a = ws + (size_t)tot * kLimb;
   uint32_t *rest = ws + 2 * (size_t)tot * kLimb;
-  const ModDownPlan *fold = fold_plan(c, level, rescale ? 2 : 0);
   int rc = op_mult_key_no_moddown(c, level, acc_b, acc_a, d2, d0, d1, key_b,
-                                  key_a, rest, s, nullptr, fold);
+                                  key_a, rest, s);
   if (rc) return rc;
   uint32_t *dst[2] = {out_b, out_a};
   const uint32_t *src[2] = {acc_b, acc_a};
-  return op_moddown(c, level, rescale ? 2 : 0, 2, dst, src, rest, s, fold != nullptr);
+  return op_moddown(c, level, rescale ? 2 : 0, 2, dst, src, rest, s);
 }
 
 // HMult (Context.cpp:862-871): Tensor then Relinearize(Rescale), with the
@@ -519,13 +504,12 @@ int op_hmult(const Context &c, int level, uint32_t *out_b, uint32_t *out_a,
   uint32_t *acc_a = ws + (size_t)tot * kLimb;
   uint32_t *rest = ws + 2 * (size_t)tot * kLimb;
   const uint32_t *tensor[4] = {b1, a1, b2, a2};
-  const ModDownPlan *fold = fold_plan(c, level, rescale ? 2 : 0);
   int rc = op_mult_key_no_moddown(c, level, acc_b, acc_a, nullptr, nullptr,
-                                  nullptr, key_b, key_a, rest, s, tensor, fold);
+                                  nullptr, key_b, key_a, rest, s, tensor);
   if (rc) return rc;
   uint32_t *dst[2] = {out_b, out_a};
   const uint32_t *src[2] = {acc_b, acc_a};
-  return op_moddown(c, level, rescale ? 2 : 0, 2, dst, src, rest, s, fold != nullptr);
+  return op_moddown(c, level, rescale ? 2 : 0, 2, dst, src, rest, s);
 }
 
 // MultKey on (b, a) (Context.cpp:584-607): key-switch a, accum.b += P b,
@@ -539,13 +523,12 @@ int op_mult_key(const Context &c, int level, uint32_t *out_b, uint32_t *out_a,
   uint32_t *acc_b = ws;
   uint32_t *acc_a = ws + (size_t)tot * kLimb;
   uint32_t *rest = ws + 2 * (size_t)tot * kLimb;
-  const ModDownPlan *fold = fold_plan(c, level, 0);
   int rc = op_mult_key_no_moddown(c, level, acc_b, acc_a, a, b, nullptr, key_b,
-                                  key_a, rest, s, nullptr, fold);
+                                  key_a, rest, s);
   if (rc) return rc;
   uint32_t *dst[2] = {out_b, out_a};
   const uint32_t *src[2] = {acc_b, acc_a};
-  return op_moddown(c, level, 0, 2, dst, src, rest, s, fold != nullptr);
+  return op_moddown(c, level, 0, 2, dst, src, rest, s);
 }
 
 // HRot / HConj (Context.cpp:776-798): MultKey then Permute by galois factor

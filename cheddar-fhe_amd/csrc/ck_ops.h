@@ -80,23 +80,16 @@ int op_modup(const Context &c, int level, uint32_t *dst, const uint32_t *src,
              uint32_t *v, Sink &s, bool copy_own = true,
              const uint32_t *prod2 = nullptr, bool cols_only = false);
 const ModDownPlan *moddown_plan(const Context &c, int level, int kind);
-// rows_done: the sources' removed limbs already hold the INTT's row-stage
-// output (key_rows did them), so only the INTT's column pass runs
 int op_moddown(const Context &c, int level, int kind, int npoly,
                uint32_t *const *dst, const uint32_t *const *src, uint32_t *v,
-               Sink &s, bool rows_done = false);
+               Sink &s);
 size_t mult_key_ws_limbs(const Context &c, int level);
 int op_mult_key_no_moddown(const Context &c, int level, uint32_t *acc_b,
                            uint32_t *acc_a, const uint32_t *x,
                            const uint32_t *add_b, const uint32_t *add_a,
                            const uint32_t *const *key_b,
                            const uint32_t *const *key_a, uint32_t *ws,
-                           Sink &s, const uint32_t *const *tensor = nullptr,
-                           const ModDownPlan *fold_inv_rows = nullptr);
-// the ModDown that follows a key switch at this level: the key inner
-// product pass can apply its INTT's row stages (fused key pass only); the
-// return value is the plan to pass as fold_inv_rows, or null
-const ModDownPlan *fold_plan(const Context &c, int level, int kind);
+                           Sink &s, const uint32_t *const *tensor = nullptr);
 size_t relin_ws_limbs(const Context &c, int level);
 int op_relinearize(const Context &c, int level, uint32_t *out_b,
                    uint32_t *out_a, const uint32_t *d0, const uint32_t *d1,

@@ -48,7 +48,6 @@ PlanStage &PlanSink::stage(PlanStage::Kind kind) {
 int PlanSink::pass(bool forward, int flags, const std::vector<PassJob> &jobs) {
   PlanStage &s = stage(forward ? PlanStage::kFwd : PlanStage::kInv);
   s.flags_any |= flags;
-  s.flags_all &= flags;
   for (const PassJob &j : jobs) {
     LimbJob J;
     std::memset(&J, 0, sizeof(J));
@@ -216,7 +215,7 @@ int plan_run(const Plan &p, hipStream_t st) {
         launch_ntt_fwd_table(c, s.d_jobs, (int)s.jobs.size(), s.flags_any, s.max_bs, st);
         break;
       case PlanStage::kInv:
-        launch_intt_table(c, s.d_jobs, (int)s.jobs.size(), s.flags_any, s.flags_all, st);
+        launch_intt_table(c, s.d_jobs, (int)s.jobs.size(), s.flags_any, st);
         break;
       case PlanStage::kKeyAccum:
         launch_key_accum_table(c, s.d_ka, s.d_ew, (int)s.ew.size(), st);
@@ -297,7 +296,7 @@ int plan_launches(const Plan &p) {
   int n = 0;
   for (const PlanStage &s : p.stages)
     n += ((s.kind == PlanStage::kFwd && !(s.flags_any & kNttColsOnly)) ||
-          (s.kind == PlanStage::kInv && !(s.flags_all & kInttColsOnly)))
+          s.kind == PlanStage::kInv)
              ? 2
              : 1;
   return n;

@@ -15,7 +15,6 @@ struct PlanStage {
     kBconvI8, kBconvValu    // BaseConv on the int8 matrix cores / vector unit
   } kind = kFwd;
   int flags_any = 0;
-  int flags_all = -1;  // AND of the jobs' flags
   int max_bs = 0;  // largest BaseConv source count among the jobs
   std::vector<LimbJob> jobs;
   std::vector<KeyAccumItem> ka;
