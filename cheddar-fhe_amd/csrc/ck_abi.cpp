@@ -995,10 +995,19 @@ int ck_plan_ntt(ck_context *ctx, int n, int num_main, int num_ter, int num_aux,
   CK_REQUIRE(n >= 1, "a plan needs at least one polynomial");
   CK_REQUIRE(valid_np(c, num_main, num_ter, num_aux), "invalid NP");
   for (int k = 0; k < n; k++) CK_REQUIRE(src[k] && dst[k], "null pointer");
-  // the int8 matrix-core form covers the forward transform
-  const int form = inverse ? 0 : c.plan_ntt_form;
-  CK_REQUIRE(form >= 0 || !aliases((const void *const *)src, n, (const void *const *)dst, n),
-             "autotuned plans need outputs that do not alias inputs");
+  // the int8 matrix-core form covers the forward transform; its column pass
+  // writes a tiled intermediate over dst, which must therefore not overlap
+  // any input: in-place plans take the vector form
+  bool overlap = false;
+  const size_t bytes = (size_t)(num_main + num_ter + num_aux) * kLimb * sizeof(uint32_t);
+  for (int k = 0; k < n && !overlap; k++)
+    for (int j = 0; j < n && !overlap; j++) {
+      const char *d0 = (const char *)dst[k], *s0 = (const char *)src[j];
+      overlap = d0 < s0 + bytes && s0 < d0 + bytes;
+    }
+  int form = inverse ? 0 : c.plan_ntt_form;
+  CK_REQUIRE(form >= 0 || !overlap, "autotuned plans need outputs that do not alias inputs");
+  if (overlap) form = 0;
   const int limbs = num_main + num_ter + num_aux;
   if (form != 0) {
     I8NttTabs T;

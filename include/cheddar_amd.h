@@ -348,7 +348,9 @@ int ck_plan_bconv(ck_context *ctx, const int *src_primes, int n_src,
                   uint32_t *const *dst, ck_plan **out);
 /* dst[k] = ck_ntt (inverse = 0) or ck_intt (inverse = 1) of src[k], k < n,
  * all with one NP (NTTHandler::NTT / INTT over a batch): one launch per pass
- * across the batch. */
+ * across the batch. The int8 form's column pass writes a tiled intermediate
+ * over dst, so a plan whose outputs overlap its inputs (in place) uses the
+ * vector form. */
 int ck_plan_ntt(ck_context *ctx, int n, int num_main, int num_ter, int num_aux,
                 int inverse, int montgomery_conversion, const uint32_t *const *src,
                 uint32_t *const *dst, ck_plan **out);

@@ -212,3 +212,29 @@ def test_bconv_autotune(ck, ctx24, oracle_rns24):
         assert plan.info()["form"] == min(tuning, key=lambda t: t[3])[0]
     finally:
         ctx24.set_plan_forms(bconv="int8")
+
+
+def test_ntt_plan_in_place(ck, ctx24, oracle_rns24):
+    """in place (dst = src): the int8 column pass would overwrite inputs with
+    its tiled intermediate, so the plan takes the vector form; an autotuned
+    in-place plan is refused"""
+    P = oracle_rns24
+    np3 = (24, 0, 8)
+    pidx = P.np_primes(*np3)
+    xs = [plant_extremes(P, uniform(P, pidx, SEED_BASE + 3 * 1000 + 400 + k), pidx)
+          for k in range(3)]
+    d = [ck.to_device(x) for x in xs]
+    ctx24.set_plan_forms(ntt="int8")
+    try:
+        plan = ck.Plan.ntt(ctx24, d, d, np3, inverse=False, mont_conv=True)
+        assert plan.info()["form"] == 0
+        plan.run()
+        sync()
+        for k in range(3):
+            assert_same(ck.to_host(d[k]), P.ntt_np(xs[k], *np3, mont_conv=True),
+                        f"in-place NTT poly {k}")
+        ctx24.set_plan_forms(ntt="auto")
+        with pytest.raises(ck.CheddarError):
+            ck.Plan.ntt(ctx24, d, d, np3, inverse=False)
+    finally:
+        ctx24.set_plan_forms(ntt="int8")
