@@ -360,8 +360,9 @@ def run_batched(torch, ck, steps, warmup, autotune=True, bconv="auto", check=4):
             "ms_hmult_batch": t_mult, "ms_hrot_batch": t_rot, "steps": steps,
             "levels": levels, "rotations": rots, "autotune": autotune,
             "hmult_plan": p_mult.info(), "hrot_plan": p_rot.info(),
-            "hmult_plan_bconv_us": {("fused", "fp64", "int8")[f]: us
-                                    for f, _, _, us in p_mult.tuning()},
+            "hmult_plan_candidates_us": {
+                ("fused", "fp64", "int8")[f] + ("+key_rows" if kr else "+key_accum"): us
+                for f, kr, _, us in p_mult.tuning()},
             "hrot_plan_bconv_us": {("fused", "fp64", "int8")[f]: us
                                    for f, _, _, us in p_rot.tuning()},
             "checked_items": sample, "equal_to_single_ops": ok}

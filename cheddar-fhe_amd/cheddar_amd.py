@@ -535,6 +535,8 @@ class Plan:
     def info(self):
         out = (ctypes.c_longlong * 8)()
         _check(lib.ck_plan_info(self._h, out))
+        # form: a transform / BaseConv plan's form, or an HMult plan's key
+        # pass (1: ModUp row stages fused with the key product, 0: separate)
         return {"items": out[0], "steps": out[1], "launches": out[2],
                 "workspace_bytes": out[3], "bconv_mfma": bool(out[4]),
                 "bconv_form": ("fused", "fp64", "int8")[out[4]] if out[4] < 3 else None,
@@ -542,7 +544,9 @@ class Plan:
                 "form": out[7]}
 
     def tuning(self):
-        """[(form, int8 blocks, int8 chunk, us)] of every autotuned candidate"""
+        """[(form, int8 blocks, int8 chunk, us)] of every autotuned candidate of
+        a transform / BaseConv plan, [(BaseConv form, key pass, 0, us)] of an
+        HMult / hoisted plan"""
         k = lib.ck_plan_tuning(self._h, 0, None, None)
         if k < 0:
             raise CheddarError(lib.ck_last_error().decode())

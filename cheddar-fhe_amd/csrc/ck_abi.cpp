@@ -733,32 +733,39 @@ int ck_hrot_hoisted(ck_context *ctx, int level, int n_rot, const int *rots,
                               key_a, moddown != 0, kMaxBsRot, c.ws, sink);
 }
 
-// Builds a plan with the context's BaseConv form, or with autotuning all
-// three (fused into the forward column pass, on the FP64 matrix cores, on
-// the int8 matrix cores), times each and keeps the fastest.
+// Builds a plan with the context's BaseConv form and key pass, or with
+// autotuning all three BaseConv forms (fused into the forward column pass,
+// on the FP64 matrix cores, on the int8 matrix cores) and, when key_dim,
+// both key passes (the ModUp's row stages fused with the key inner product,
+// or a row pass plus a separate accumulation): times each and keeps the
+// fastest. build(bconv form, key pass (-1: the context's), &plan).
 extern "C++" template <typename Build>
-static int build_tuned(const Context &c, Build &build, Plan **out) {
+static int build_tuned(const Context &c, Build &build, bool key_dim, Plan **out) {
   *out = nullptr;
-  if (!c.plan_autotune) return build(c.bconv_mfma < 0 ? 2 : c.bconv_mfma, out);
+  if (!c.plan_autotune) return build(c.bconv_mfma < 0 ? 2 : c.bconv_mfma, -1, out);
   Plan *best = nullptr;
+  double best_us = -1.0;
   double us[3] = {-1.0, -1.0, -1.0};
   std::vector<Plan::Candidate> rec;
-  for (int f = 0; f < 3; f++) {
-    Plan *p = nullptr;
-    int rc = build(f, &p);
-    if (rc) {
-      plan_destroy(best);
-      return rc;
+  for (int f = 0; f < 3; f++)
+    for (int kr : key_dim ? std::vector<int>{1, 0} : std::vector<int>{-1}) {
+      Plan *p = nullptr;
+      int rc = build(f, kr, &p);
+      if (rc) {
+        plan_destroy(best);
+        return rc;
+      }
+      const double t = plan_time_us(*p, 3);
+      rec.push_back({f, kr < 0 ? (c.fuse_key_rows ? 1 : 0) : kr, 0, t});
+      if (t >= 0 && (us[f] < 0 || t < us[f])) us[f] = t;
+      if (t >= 0 && (!best || t < best_us)) {
+        plan_destroy(best);
+        best = p;
+        best_us = t;
+      } else {
+        plan_destroy(p);
+      }
     }
-    us[f] = plan_time_us(*p, 3);
-    rec.push_back({f, 0, 0, us[f]});
-    if (us[f] >= 0 && (!best || us[f] < us[best->bconv_form])) {
-      plan_destroy(best);
-      best = p;
-    } else {
-      plan_destroy(p);
-    }
-  }
   if (!best) return fail(CK_ERR_HIP, "autotuning could not time any candidate plan");
   for (int f = 0; f < 3; f++) best->tuned_us[f] = us[f];
   best->tuning = rec;
@@ -803,12 +810,13 @@ int ck_plan_hmult(ck_context *ctx, int n, const int *levels, int rescale,
     off[i] = total;
     total += relin_ws_limbs(c, levels[i]);
   }
-  auto build = [&](int form, Plan **res) {
+  auto build = [&](int form, int key_rows, Plan **res) {
     Plan *p = new Plan;
     p->ctx = &c;
     p->items = n;
     p->bconv_form = form;
     p->bconv_mfma = form != 0;
+    p->key_rows = key_rows >= 0 ? key_rows : (c.fuse_key_rows ? 1 : 0);
     uint32_t *ws = nullptr;
     int rc = plan_alloc_workspace(*p, total, &ws);
     PlanSink sink(*p);
@@ -826,7 +834,7 @@ int ck_plan_hmult(ck_context *ctx, int n, const int *levels, int rescale,
     return rc;
   };
   Plan *p = nullptr;
-  const int rc = build_tuned(c, build, &p);
+  const int rc = build_tuned(c, build, true, &p);
   if (rc) return rc;
   *out = new ck_plan{p};
   return CK_OK;
@@ -1104,12 +1112,13 @@ int ck_plan_hrot_hoisted(ck_context *ctx, int n, const int *levels,
   for (int r = 0; r < n_rot; r++)
     CK_REQUIRE(rots[r] >= 1 && rots[r] < c.n / 2,
                "hoisted rotation amounts must be in [1, N/2)");
-  auto build = [&](int form, Plan **res) {
+  auto build = [&](int form, int key_rows, Plan **res) {
     Plan *p = new Plan;
     p->ctx = &c;
     p->items = n;
     p->bconv_form = form;
     p->bconv_mfma = form != 0;
+    p->key_rows = key_rows;
     uint32_t *ws = nullptr;
     int rc = plan_alloc_workspace(*p, total, &ws);
     PlanSink sink(*p);
@@ -1128,7 +1137,7 @@ int ck_plan_hrot_hoisted(ck_context *ctx, int n, const int *levels,
     return rc;
   };
   Plan *p = nullptr;
-  const int rc = build_tuned(c, build, &p);
+  const int rc = build_tuned(c, build, false, &p);
   if (rc) return rc;
   *out = new ck_plan{p};
   return CK_OK;
@@ -1148,7 +1157,7 @@ int ck_plan_info(const ck_plan *plan, long long *out) {
   out[4] = plan->p->bconv_mfma ? plan->p->bconv_form : 0;
   out[5] = (long long)(plan->p->tuned_us[0] * 1000.0);  // ns, -1000: not tuned
   out[6] = (long long)(plan->p->tuned_us[1] * 1000.0);
-  out[7] = plan->p->form;
+  out[7] = plan->p->key_rows >= 0 ? plan->p->key_rows : plan->p->form;
   return CK_OK;
 }
 

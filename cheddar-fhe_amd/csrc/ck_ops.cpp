@@ -431,7 +431,7 @@ int op_mult_key_no_moddown(const Context &c, int level, uint32_t *acc_b,
   const int naux = level_naux(c, level);
   uint32_t *v = ws;
   uint32_t *modup = ws + (size_t)lp.num_q * kLimb;
-  const bool fused = c.fuse_key_rows;
+  const bool fused = s.fused_key_rows(c);
   int rc = op_modup(c, level, modup, tensor ? tensor[1] : x, v, s, false,
                     tensor ? tensor[3] : nullptr, fused);
   if (rc) return rc;

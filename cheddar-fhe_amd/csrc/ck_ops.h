@@ -39,6 +39,9 @@ class Sink {
   virtual int bconv(const std::vector<BconvInst> &insts) = 0;
   // whether ops should emit bconv() steps rather than fused BaseConv passes
   virtual bool mfma_bconv() const = 0;
+  // key switches: the ModUp's forward row stages fused with the key inner
+  // product (key_rows) or a row pass plus key_accum
+  virtual bool fused_key_rows(const Context &c) const { return c.fuse_key_rows; }
   virtual int key_accum(const EwNp &np, const KeyAccumArgs &A) = 0;
   // forward row stages of a column-pass-only ModUp fused with key_accum
   virtual int key_rows(const EwNp &np, const KeyAccumArgs &A) = 0;

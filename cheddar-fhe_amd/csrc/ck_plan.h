@@ -41,6 +41,7 @@ struct Plan {
   int items = 0;
   bool bconv_mfma = true;  // BaseConv steps on the matrix cores
   int bconv_form = 2;       // 0 fused, 1 FP64 matrix cores, 2 int8 matrix cores
+  int key_rows = -1;        // key pass: 1 fused row stages, 0 separate, -1 the context's
   // autotuning record: measured run time of each BaseConv form (us; < 0:
   // not measured): fused, FP64 matrix cores, int8 matrix cores
   double tuned_us[3] = {-1.0, -1.0, -1.0};
@@ -61,6 +62,9 @@ class PlanSink : public Sink {
  public:
   explicit PlanSink(Plan &p) : plan_(p) {}
   bool mfma_bconv() const override { return plan_.bconv_mfma; }
+  bool fused_key_rows(const Context &c) const override {
+    return plan_.key_rows >= 0 ? plan_.key_rows != 0 : c.fuse_key_rows;
+  }
   void begin_item() { cursor_ = 0; }
   bool mismatch() const { return mismatch_; }
   int pass(bool forward, int flags, const std::vector<PassJob> &jobs) override;

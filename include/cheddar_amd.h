@@ -360,11 +360,15 @@ int ck_plan_run(ck_plan *plan, void *stream);
  * matrix cores), with
  * CK_OPT_PLAN_AUTOTUNE the measured run time of each of those forms in ns
  * (fused, matrix cores; -1000 when not tuned), and the form of a transform
- * or BaseConv plan (CK_OPT_PLAN_NTT_FORM / CK_OPT_PLAN_BCONV_FORM values) */
+ * or BaseConv plan (CK_OPT_PLAN_NTT_FORM / CK_OPT_PLAN_BCONV_FORM values) or
+ * the key pass of an HMult plan (1: the ModUp's row stages fused with the
+ * key inner product, 0: a row pass plus a separate accumulation) */
 int ck_plan_info(const ck_plan *plan, long long *out);
 /* Autotuning record of a plan built with an autotuned form option: for up
  * to max candidates, forms[3 i .. 3 i + 2] = (form, int8 blocks per
- * workgroup, int8 chunk in limbs) and us[i] = its measured run time in
+ * workgroup, int8 chunk in limbs) of a transform or BaseConv plan, or
+ * (BaseConv form, key pass (as in ck_plan_info), 0) of an HMult / hoisted
+ * plan, and us[i] = its measured run time in
  * microseconds. Returns the number of candidates (0: not autotuned), or -1
  * on error. No reference counterpart (the reference fixes its launch
  * shapes at compile time, include/core/NTTUtils.cuh:395-437,
@@ -383,7 +387,8 @@ int ck_plan_destroy(ck_plan *plan);
 #define CK_OPT_BCONV_MFMA 1
 /* CK_OPT_PLAN_AUTOTUNE: 1 makes ck_plan_hmult / ck_plan_hrot_hoisted build
  * the plan with each BaseConv form (fused, FP64 matrix cores, int8 matrix
- * cores), run each (one warm-up + 3 timed runs on a private stream,
+ * cores; for ck_plan_hmult each with both key passes, CK_OPT_FUSED_KEY_ROWS
+ * 1 and 0), run each (one warm-up + 3 timed runs on a private stream,
  * synchronising the host: plan creation only; the runs write the plan's
  * outputs) and keep the fastest (ck_plan_tuning lists the times); 0
  * (default) uses CK_OPT_BCONV_MFMA. */

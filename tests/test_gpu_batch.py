@@ -63,14 +63,17 @@ def test_plan_hmult_mixed_levels(ck, ctx24, oracle_rns24, key24):
     info = plan.info()
     # 9 launches (BaseConv fused) or 11 (two BaseConv steps on the matrix
     # cores); one fewer with the key product fused into the ModUp's row stages
+    # (info "form": the plan's key pass)
     assert info["items"] == len(levels)
-    assert info["launches"] == ((11 if info["bconv_mfma"] else 9)
-                                - (1 if ctx24.fused_key_rows else 0))
     tuning = plan.tuning()
-    if tuning:  # autotuned: all three BaseConv forms timed, the fastest kept
-        assert sorted(t[0] for t in tuning) == [0, 1, 2] and min(t[3] for t in tuning) > 0
-        assert ("fused", "fp64", "int8")[min(tuning, key=lambda t: t[3])[0]] == \
-            info["bconv_form"]
+    if not tuning:
+        assert info["form"] == int(ctx24.fused_key_rows)
+    assert info["launches"] == ((11 if info["bconv_mfma"] else 9) - info["form"])
+    if tuning:  # autotuned: every BaseConv form with both key passes, fastest kept
+        assert sorted((t[0], t[1]) for t in tuning) == [(f, k) for f in range(3) for k in (0, 1)]
+        assert min(t[3] for t in tuning) > 0
+        best = min(tuning, key=lambda t: t[3])
+        assert ("fused", "fp64", "int8")[best[0]] == info["bconv_form"] and best[1] == info["form"]
     for rep in range(2):
         for o in outs:
             o[0].fill_(-1)
