@@ -110,14 +110,19 @@ _KERNELS = ("ntt_fwd_cols", "intt_cols", "ntt_fwd_rows", "intt_rows", "key_accum
             "key_accum_table", "bs_fused", "bs_fused_table", "tensor", "permute",
             "binary", "unary", "const", "accum", "imag_unit", "crt_lift", "bconv_mfma",
             "bconv_mfma_table", "key_rows", "key_rows_table", "gs_accum")
+# int8 matrix-core kernels (batched plans): labelled by workgroup count, since
+# their workgroups per limb depend on the plan's blocks per workgroup
+_I8_KERNELS = ("ntt8_cols", "ntt8_rows", "bconv_i8", "bconv_valu")
 
 
 def _limbs(short, grid_x, grid_y, wg_x):
     """limbs of one launch from its grid (None for kernels not of this
     library)"""
+    wgs = grid_x // max(1, wg_x)
+    if short in _I8_KERNELS:
+        return f"{wgs}wg"
     if short not in _KERNELS:
         return None
-    wgs = grid_x // max(1, wg_x)
     if short in ("ntt_fwd_cols", "intt_cols"):
         return wgs // 16
     if short in ("bs_fused", "bs_fused_table"):
