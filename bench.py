@@ -206,6 +206,29 @@ def copy_calibration(torch):
     log("copy calibration: 268435456 bytes copied")
 
 
+def device_copy_bandwidth(torch, reps=10):
+    """A plain device-to-device copy of 1 GiB (1 GiB read + 1 GiB written),
+    timed over `reps` copies with HIP events on the current stream after one
+    warm-up: the HBM rate a streaming kernel reaches on this GPU, reported
+    beside the published peak."""
+    src = torch.ones(256 << 20, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    moved = 2 * src.numel() * 4
+    del src, dst
+    return {"gb_s": moved / (ms * 1e-3) / 1e9, "bytes_per_copy": moved, "ms_per_copy": ms,
+            "frac_of_peak": moved / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -499,6 +522,10 @@ def main():
     log(f"dominant kernel {dom}: {launch_us:.2f} us/launch, {alg_bytes / 1e6:.1f} MB "
         f"algorithmic -> {achieved:.0f} GB/s")
     traffic = measured_traffic(dom)
+    copy_bw = device_copy_bandwidth(torch)
+    log(f"device copy: {copy_bw['gb_s']:.0f} GB/s ({copy_bw['frac_of_peak']:.2f} of the "
+        f"{HBM_PEAK_GBS:.0f} GB/s peak); dominant kernel at "
+        f"{achieved / copy_bw['gb_s']:.2f} of the copy rate")
     if a.copy_calibration:
         copy_calibration(torch)
 
@@ -509,6 +536,9 @@ def main():
         log(f"cpu baseline: {cpu['value']:.3f} HMult/s")
 
     config3 = bench_config3(torch, ck, ctx)
+    for k in ("bconv", "ntt"):
+        config3[f"{k}_frac_of_device_copy"] = (config3[f"{k}_hbm_frac"] * HBM_PEAK_GBS /
+                                              copy_bw["gb_s"])
     batched = None
     if not a.no_batched:
         batched = run_batched(torch, ck, steps=10, warmup=2)
@@ -534,7 +564,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "launch_us": launch_us,
-                     "alg_bytes_per_launch": alg_bytes},
+                     "alg_bytes_per_launch": alg_bytes,
+                     "frac_of_device_copy": achieved / copy_bw["gb_s"]},
+        "device_copy": copy_bw,
         "cpu_baseline": cpu,
         "config3": config3,
         "batched": batched,
