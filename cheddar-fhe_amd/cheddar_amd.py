@@ -82,6 +82,7 @@ _SIGS = {
     "ck_tensor": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int] + [_vp] * 7 + [_vp]),
     "ck_permute": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, _vp]),
+    "ck_copy": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t, _vp]),
     "ck_mult_key_no_moddown": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
                                               ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
     "ck_mult_key": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp,
@@ -332,6 +333,11 @@ class Context:
         flat = [v for term in xs for v in term]
         _check(lib.ck_paccum(self._h, *np3, len(dsts), len(ys), _ptrs(dsts), _ptrs(flat),
                              _ptrs(ys), int(accumulate), _stream(stream)))
+
+    def copy(self, dst, src, nbytes=None, stream=None):
+        """device copy of nbytes bytes (default: all of src) from src to dst"""
+        n = src.numel() * src.element_size() if nbytes is None else int(nbytes)
+        _check(lib.ck_copy(self._h, _ptr(dst), _ptr(src), n, _stream(stream)))
 
     def gs_accum(self, dsts, xs, pts, np3, stream=None):
         """dsts[g] = [sum_t xs[t][j] * pts[g][t] for j in (b, a)] for every

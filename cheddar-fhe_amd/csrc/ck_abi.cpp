@@ -281,6 +281,15 @@ int ck_moddown(ck_context *ctx, int level, int kind, int npoly,
   return op_moddown(c, level, kind, npoly, dst, src, c.ws, sink);
 }
 
+int ck_copy(ck_context *ctx, void *dst, const void *src, size_t bytes, void *stream) {
+  CK_REQUIRE(ctx && ctx->c, "null context");
+  if (bytes == 0) return CK_OK;
+  CK_REQUIRE(dst && src, "null pointer");
+  const char *d = static_cast<const char *>(dst), *s = static_cast<const char *>(src);
+  CK_REQUIRE(d == s || d + bytes <= s || s + bytes <= d, "copy ranges overlap");
+  return launch_copy(*ctx->c, dst, src, bytes, as_stream(stream));
+}
+
 int ck_add(ck_context *ctx, int num_main, int num_ter, int num_aux, int npoly,
            uint32_t *const *dst, const uint32_t *const *x,
            const uint32_t *const *y, void *stream) {
@@ -671,13 +680,9 @@ int ck_hrot(ck_context *ctx, int level, uint32_t *out_b, uint32_t *out_a,
   const size_t bytes = (size_t)c.levels[level].num_q * kLimb * 4;
   if (rot == 0) {
     // Context::HRot copies when the rotation is a no-op
-    int rc = check_hip(hipMemcpyAsync(out_b, b, bytes, hipMemcpyDeviceToDevice,
-                                      as_stream(stream)),
-                       "hrot copy");
+    const int rc = launch_copy(c, out_b, b, bytes, as_stream(stream));
     if (rc) return rc;
-    return check_hip(hipMemcpyAsync(out_a, a, bytes, hipMemcpyDeviceToDevice,
-                                    as_stream(stream)),
-                     "hrot copy");
+    return launch_copy(c, out_a, a, bytes, as_stream(stream));
   }
   CK_REQUIRE(valid_key(c, key_b, level) && valid_key(c, key_a, level),
              "null key block");

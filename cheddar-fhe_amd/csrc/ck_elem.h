@@ -163,6 +163,11 @@ void launch_gs_accum(const Context &ctx, const EwNp &np, const GsAccumArgs &A,
                      hipStream_t st);
 void launch_permute_accum(const Context &ctx, const EwNp &np, const PermAccArgs &A,
                           hipStream_t st);
+// bytes from src to dst (ranges equal or disjoint) on the device: 16-byte
+// vectors when both are 16-byte aligned and bytes % 16 == 0, else
+// hipMemcpyAsync; returns a status code
+int launch_copy(const Context &ctx, void *dst, const void *src, size_t bytes,
+                hipStream_t st);
 void launch_tensor(const Context &ctx, const EwNp &np, const EwOperand &c1,
                    const EwOperand &c2, const EwOperand &out, hipStream_t st);
 void launch_key_accum(const Context &ctx, const EwNp &np,

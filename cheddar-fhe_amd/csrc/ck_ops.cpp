@@ -110,9 +110,7 @@ int ImmediateSink::pass(bool forward, int flags,
 }
 
 int ImmediateSink::copy(uint32_t *dst, const uint32_t *src, size_t words) {
-  return check_hip(hipMemcpyAsync(dst, src, words * sizeof(uint32_t),
-                                  hipMemcpyDeviceToDevice, st_),
-                   "copy");
+  return launch_copy(c_, dst, src, words * sizeof(uint32_t), st_);
 }
 
 int ImmediateSink::bconv(const std::vector<BconvInst> &insts) {

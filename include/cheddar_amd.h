@@ -28,6 +28,7 @@
 #ifndef CHEDDAR_AMD_H_
 #define CHEDDAR_AMD_H_
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -235,6 +236,13 @@ int ck_tensor(ck_context *ctx, int num_main, int num_ter, uint32_t *d0,
 int ck_permute(ck_context *ctx, int num_main, int num_ter, int num_aux,
                int npoly, uint32_t *const *dst, const uint32_t *const *x,
                int rot, void *stream);
+/* Device-to-device copy of `bytes` bytes (DeviceVector's device copies,
+ * src/core/DeviceVector.cpp:105-124; ModUp's copy of a group's own limbs,
+ * src/core/ModSwitch.cu:467-469, uses the same kernel): 16-byte vectors
+ * with non-temporal accesses when dst and src are 16-byte aligned and bytes
+ * is a multiple of 16, else hipMemcpyAsync. dst == src is a no-op; ranges
+ * that overlap otherwise are rejected. */
+int ck_copy(ck_context *ctx, void *dst, const void *src, size_t bytes, void *stream);
 
 /* ---- Context evaluator (include/core/Context.h:101-423) --------------- */
 /* Ciphertext polynomials are num_q(level) contiguous limbs; keys are

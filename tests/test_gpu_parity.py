@@ -417,6 +417,34 @@ def test_permute(ck, ctx24, oracle_rns24, rot):
     assert_same(ck.to_host(out), want, f"permute {rot}")
 
 
+def test_copy(ck, ctx24):
+    """ck_copy (DeviceVector's device copies, src/core/DeviceVector.cpp:
+    105-124): the 16-byte vector kernel for aligned ranges whose size is a
+    multiple of 16 (one limb, three limbs and 16 bytes, 64 MiB), the
+    runtime's copy for the others (12 bytes; an offset of 4 bytes; 1000
+    bytes); bytes outside the range stay untouched, dst == src is a no-op
+    and overlapping ranges are refused"""
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(SEED_BASE + 90)
+    for nbytes, off in [(1 << 18, 0), (3 << 18 | 16, 0), (64 << 20, 0), (12, 0),
+                        (1 << 18, 4), (1000, 4)]:
+        total = nbytes + off + 64
+        src = torch.randint(0, 256, (total,), generator=gen, device="cuda",
+                            dtype=torch.int32).to(torch.uint8)
+        dst = torch.zeros(total, dtype=torch.uint8, device="cuda")
+        ctx24.copy(dst[off:], src[off:], nbytes)
+        sync()
+        got, want = dst.cpu(), src.cpu()
+        assert torch.equal(got[off:off + nbytes], want[off:off + nbytes]), (nbytes, off)
+        assert int(got[:off].sum()) == 0 and int(got[off + nbytes:].sum()) == 0, (nbytes, off)
+    buf = torch.arange(4096, device="cuda", dtype=torch.int32).to(torch.uint8)
+    ctx24.copy(buf, buf, 4096)
+    sync()
+    assert torch.equal(buf.cpu(), torch.arange(4096, dtype=torch.int32).to(torch.uint8))
+    with pytest.raises(ck.CheddarError):
+        ctx24.copy(buf[16:], buf[:2048], 2048)
+
+
 @pytest.mark.parametrize("accumulate", [False, True])
 def test_permute_accum(ck, ctx24, oracle_rns24, accumulate):
     """sum of automorphisms over Q*P limbs (PermuteAccum, src/core/
@@ -503,7 +531,8 @@ def test_mult_key_no_moddown(ck, ctx24, oracle_rns24, key24):
     assert_same(ck.to_host(acc[1]), want[1], "accum a")
 
 
-@pytest.mark.parametrize("level,rot", [(23, 1), (23, 1234), (12, 32767), (3, 5)])
+@pytest.mark.parametrize("level,rot", [(23, 1), (23, 1234), (12, 32767), (3, 5), (23, 0),
+                                       (9, 32768)])
 def test_hrot(ck, ctx24, oracle_rns24, key24, level, rot):
     P = oracle_rns24
     pidx = P.level_primes(level)
