@@ -160,3 +160,34 @@ def test_autotuned_plans_reject_aliasing(ck, rns24, oracle_rns24, key24):
         ck.Plan.hmult(ctx, [9], [[ct[0], ct[1]]], [ct], [other], dev_key(*key24))
     with pytest.raises(ck.CheddarError):
         ck.Plan.hrot_hoisted(ctx, [9], [ct], [dev_key(*key24)], [5], [[[ct[0], other[1]]]])
+
+
+@pytest.mark.parametrize("form", ["bconv-fused", "bconv-int8"])
+def test_plan_hmult_terminal_primes(ck, boot40, oracle_boot40, form):
+    """HMult plan on bootparam_40 at levels whose ModDownAndRescale drops
+    terminal primes (28 -> 27), drops main primes and restores terminal ones
+    (3 -> 2), drops terminal primes and restores main ones (2 -> 1), and a
+    plain main-prime level: the restore limbs' epilogue takes the padding
+    constant, and limbs with no second operand take zero; BaseConv fused into
+    the forward column pass, or on the int8 matrix cores as its own step."""
+    P = oracle_boot40
+    ctx = ck.Context.from_json(boot40)
+    ctx.set_bconv_mfma("int8" if form == "bconv-int8" else False)
+    key = full_key(P, SEED_BASE + 2000)
+    levels = [28, 3, 2, 9]
+    cts1, cts2, outs = [], [], []
+    for i, lv in enumerate(levels):
+        pidx = P.level_primes(lv)
+        cts1.append([uniform(P, pidx, SEED_BASE + 3500 + 4 * i + j) for j in range(2)])
+        cts2.append([uniform(P, pidx, SEED_BASE + 3502 + 4 * i + j) for j in range(2)])
+        outs.append(_empty(P, lv - 1))
+    d1 = [[ck.to_device(v) for v in c] for c in cts1]
+    d2 = [[ck.to_device(v) for v in c] for c in cts2]
+    plan = ck.Plan.hmult(ctx, levels, outs, d1, d2, dev_key(*key))
+    assert plan.info()["bconv_form"] == ("int8" if form == "bconv-int8" else "fused")
+    plan.run()
+    torch.cuda.synchronize()
+    for i, lv in enumerate(levels):
+        want = P.hmult(lv, cts1[i], cts2[i], key, rescale=True)
+        assert_same(ck.to_host(outs[i][0]), want[0], f"boot40 plan hmult b level {lv}")
+        assert_same(ck.to_host(outs[i][1]), want[1], f"boot40 plan hmult a level {lv}")
