@@ -121,6 +121,8 @@ def _limbs(short, grid_x, grid_y, wg_x):
     wgs = grid_x // max(1, wg_x)
     if short in _I8_KERNELS:
         return f"{wgs}wg"
+    if short == "copy16":  # ck_copy: 16 bytes per thread, labelled by size
+        return f"{(wgs * wg_x * 16) >> 20}MiB"
     if short not in _KERNELS:
         return None
     if short in ("ntt_fwd_cols", "intt_cols"):
