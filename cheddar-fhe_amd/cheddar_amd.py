@@ -335,8 +335,14 @@ class Context:
                              _ptrs(ys), int(accumulate), _stream(stream)))
 
     def copy(self, dst, src, nbytes=None, stream=None):
-        """device copy of nbytes bytes (default: all of src) from src to dst"""
-        n = src.numel() * src.element_size() if nbytes is None else int(nbytes)
+        """device copy of nbytes bytes (default: all of src) from src to dst;
+        both tensors must hold nbytes contiguous bytes"""
+        have_s = src.numel() * src.element_size()
+        have_d = dst.numel() * dst.element_size()
+        n = have_s if nbytes is None else int(nbytes)
+        if n < 0 or n > have_s or n > have_d or not (src.is_contiguous() and dst.is_contiguous()):
+            raise ValueError(f"copy of {n} bytes between tensors of {have_s} and {have_d} "
+                             "bytes (both contiguous)")
         _check(lib.ck_copy(self._h, _ptr(dst), _ptr(src), n, _stream(stream)))
 
     def gs_accum(self, dsts, xs, pts, np3, stream=None):

@@ -443,6 +443,8 @@ def test_copy(ck, ctx24):
     assert torch.equal(buf.cpu(), torch.arange(4096, dtype=torch.int32).to(torch.uint8))
     with pytest.raises(ck.CheddarError):
         ctx24.copy(buf[16:], buf[:2048], 2048)
+    with pytest.raises(ValueError):  # more bytes than the destination holds
+        ctx24.copy(buf[:1024], buf[2048:], 2048)
 
 
 @pytest.mark.parametrize("accumulate", [False, True])
