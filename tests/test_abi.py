@@ -73,3 +73,23 @@ def test_null_context_calls_fail_cleanly():
     assert ck.lib.ck_context_info(None, out) == 1
     assert ck.lib.ck_ntt(None, None, None, 1, 0, 0, 0, 0, None) == 1
     assert ck.lib.ck_hmult(None, 1, None, None, None, None, None, None, None, None, 1, None) == 1
+
+
+# entry points whose null argument is valid (a no-op) or that take no context
+_NULL_OK = {"ck_context_destroy", "ck_plan_destroy", "ck_last_error", "ck_version",
+            "ck_context_create"}
+
+
+def test_every_entry_point_refuses_a_null_context():
+    """Every entry point called with a null context (and zero / null for the
+    other arguments) reports an error instead of touching memory or the GPU:
+    status 1 (invalid argument), or -1 from the functions that return a count
+    or a level."""
+    for name, (res, args) in sorted(ck._SIGS.items()):
+        if name in _NULL_OK:
+            continue
+        vals = [None if (a in (ctypes.c_void_p, ctypes.c_char_p) or issubclass(a, ctypes._Pointer))
+                else a(0) for a in args]
+        rc = getattr(ck.lib, name)(*vals)
+        assert rc in (1, -1), (name, rc)
+        assert ck.lib.ck_last_error() is not None
