@@ -498,6 +498,67 @@ def key24(oracle_rns24):
     return full_key(oracle_rns24, SEED_BASE + 1000)
 
 
+@pytest.mark.parametrize("level", [23, 12])
+def test_keyswitch_ops_in_place(ck, ctx24, oracle_rns24, key24, level):
+    """HMult with its output over ct1's buffers (every read of ct1 precedes
+    the ModDown's first write), MultKey (Context.cpp:584-607: ModDown of the
+    key-switched a plus P b) and HRot, against the oracle"""
+    P = oracle_rns24
+    pidx = P.level_primes(level)
+    ct1 = [uniform(P, pidx, SEED_BASE + 140 + i) for i in range(2)]
+    ct2 = [uniform(P, pidx, SEED_BASE + 150 + i) for i in range(2)]
+    dk = dev_key(*key24)
+    d1 = [ck.to_device(v) for v in ct1]
+    d2 = [ck.to_device(v) for v in ct2]
+    nq, nq_out = P.num_q(level), P.num_q(level - 1)
+    want_h = P.hmult(level, ct1, ct2, key24, rescale=True)
+    acc = P.mult_key_no_moddown(level, ct1[1], key24[0], key24[1], ct1[0], None)
+    want_k = [P.moddown(acc[j], level, 0) for j in range(2)]
+    want_r = P.hrot(level, ct1, key24, 77)
+    inp = [d.clone() for d in d1]
+    ctx24.hmult([inp[0][:nq_out], inp[1][:nq_out]], inp, d2, dk, level, rescale=True)
+    outk = [torch.empty((nq, P.n), dtype=torch.int32, device="cuda") for _ in range(2)]
+    ctx24.mult_key(outk, d1, dk, level)
+    outr = [torch.empty((nq, P.n), dtype=torch.int32, device="cuda") for _ in range(2)]
+    ctx24.hrot(outr, d1, dk, 77, level)
+    sync()
+    for j in range(2):
+        assert_same(ck.to_host(inp[j][:nq_out]), want_h[j], f"in-place hmult level {level}")
+        assert_same(ck.to_host(outk[j]), want_k[j], f"mult_key level {level}")
+        assert_same(ck.to_host(outr[j]), want_r[j], f"hrot level {level}")
+
+
+def test_hmult_graph_replays(ck, ctx24, oracle_rns24, key24):
+    """one HMult captured into a HIP graph on a caller stream, replayed three
+    times with eager work and synchronisation in between: every replay
+    equals the oracle"""
+    P = oracle_rns24
+    level = 23
+    pidx = P.level_primes(level)
+    ct1 = [uniform(P, pidx, SEED_BASE + 160 + i) for i in range(2)]
+    ct2 = [uniform(P, pidx, SEED_BASE + 170 + i) for i in range(2)]
+    want = P.hmult(level, ct1, ct2, key24, rescale=True)
+    dk = dev_key(*key24)
+    d1 = [ck.to_device(v) for v in ct1]
+    d2 = [ck.to_device(v) for v in ct2]
+    out = [torch.empty((want[0].shape[0], P.n), dtype=torch.int32, device="cuda")
+           for _ in range(2)]
+    st = torch.cuda.Stream()
+    ctx24.hmult(out, d1, d2, dk, level, stream=st)  # warm-up outside capture
+    sync()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=st):
+        ctx24.hmult(out, d1, d2, dk, level, stream=st)
+    for rep in range(3):
+        for o in out:
+            o.fill_(-1)
+        sync()
+        g.replay()
+        sync()
+        for j in range(2):
+            assert_same(ck.to_host(out[j]), want[j], f"graph replay {rep} poly {j}")
+
+
 @pytest.mark.parametrize("level,rescale", [(23, True), (23, False), (8, True), (1, True)])
 def test_hmult(ck, ctx24, oracle_rns24, key24, level, rescale):
     P = oracle_rns24
