@@ -402,8 +402,9 @@ def run_batched(torch, ck, steps, warmup, autotune=True, bconv="auto", check=4):
             "levels": levels, "rotations": rots, "autotune": autotune,
             "hmult_plan": p_mult.info(), "hrot_plan": p_rot.info(),
             "hmult_plan_candidates_us": {
-                ("fused", "fp64", "int8")[f] + ("+key_rows" if kr else "+key_accum"): us
-                for f, kr, _, us in p_mult.tuning()},
+                ("fused", "fp64", "int8")[f] + ("+key_rows" if kr else "+key_accum") +
+                (f"+{ch}chunks" if ch > 1 else ""): us
+                for f, kr, ch, us in p_mult.tuning()},
             "hrot_plan_bconv_us": {("fused", "fp64", "int8")[f]: us
                                    for f, _, _, us in p_rot.tuning()},
             "checked_items": sample, "equal_to_single_ops": ok}

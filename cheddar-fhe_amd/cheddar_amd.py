@@ -557,8 +557,8 @@ class Plan:
 
     def tuning(self):
         """[(form, int8 blocks, int8 chunk, us)] of every autotuned candidate of
-        a transform / BaseConv plan, [(BaseConv form, key pass, 0, us)] of an
-        HMult / hoisted plan"""
+        a transform / BaseConv plan, [(BaseConv form, key pass, item chunks,
+        us)] of an HMult / hoisted plan"""
         k = lib.ck_plan_tuning(self._h, 0, None, None)
         if k < 0:
             raise CheddarError(lib.ck_last_error().decode())

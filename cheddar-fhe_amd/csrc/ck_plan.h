@@ -42,6 +42,7 @@ struct Plan {
   bool bconv_mfma = true;  // BaseConv steps on the matrix cores
   int bconv_form = 2;       // 0 fused, 1 FP64 matrix cores, 2 int8 matrix cores
   int key_rows = -1;        // key pass: 1 fused row stages, 0 separate, -1 the context's
+  int chunks = 1;           // item chunks, run one after another
   // autotuning record: measured run time of each BaseConv form (us; < 0:
   // not measured): fused, FP64 matrix cores, int8 matrix cores
   double tuned_us[3] = {-1.0, -1.0, -1.0};
@@ -65,7 +66,11 @@ class PlanSink : public Sink {
   bool fused_key_rows(const Context &c) const override {
     return plan_.key_rows >= 0 ? plan_.key_rows != 0 : c.fuse_key_rows;
   }
-  void begin_item() { cursor_ = 0; }
+  void begin_item() { cursor_ = base_; }
+  // the items recorded after this run as a chunk of their own: their steps
+  // are appended after the previous chunk's, so a run issues the chunks one
+  // after another
+  void begin_chunk() { base_ = plan_.stages.size(); }
   bool mismatch() const { return mismatch_; }
   int pass(bool forward, int flags, const std::vector<PassJob> &jobs) override;
   int copy(uint32_t *dst, const uint32_t *src, size_t words) override;
@@ -80,6 +85,7 @@ class PlanSink : public Sink {
   PlanStage &stage(PlanStage::Kind kind);
   Plan &plan_;
   size_t cursor_ = 0;
+  size_t base_ = 0;
   bool mismatch_ = false;
 };
 

@@ -327,7 +327,11 @@ int ck_hrot_hoisted(ck_context *ctx, int level, int n_rot, const int *rots,
  * into a HIP graph. The plan keeps the pointers it was given: buffers must
  * stay allocated until ck_plan_destroy. */
 typedef struct ck_plan ck_plan;
-/* out_* [i] = HMult(ct1[i], ct2[i]) at levels[i] with one shared key. */
+/* out_* [i] = HMult(ct1[i], ct2[i]) at levels[i] with one shared key. The
+ * items run in chunks one after another (items k, k + chunks, ... form
+ * chunk k), each chunk one launch per step: by default as many chunks as
+ * keep each chunk's workspace within 512 MiB, so that a step's outputs stay
+ * in the last-level cache for the next step. */
 int ck_plan_hmult(ck_context *ctx, int n, const int *levels, int rescale,
                   uint32_t *const *out_b, uint32_t *const *out_a,
                   const uint32_t *const *b1, const uint32_t *const *a1,
@@ -375,8 +379,8 @@ int ck_plan_info(const ck_plan *plan, long long *out);
 /* Autotuning record of a plan built with an autotuned form option: for up
  * to max candidates, forms[3 i .. 3 i + 2] = (form, int8 blocks per
  * workgroup, int8 chunk in limbs) of a transform or BaseConv plan, or
- * (BaseConv form, key pass (as in ck_plan_info), 0) of an HMult / hoisted
- * plan, and us[i] = its measured run time in
+ * (BaseConv form, key pass (as in ck_plan_info), item chunks) of an HMult /
+ * hoisted plan, and us[i] = its measured run time in
  * microseconds. Returns the number of candidates (0: not autotuned), or -1
  * on error. No reference counterpart (the reference fixes its launch
  * shapes at compile time, include/core/NTTUtils.cuh:395-437,
@@ -396,10 +400,11 @@ int ck_plan_destroy(ck_plan *plan);
 /* CK_OPT_PLAN_AUTOTUNE: 1 makes ck_plan_hmult / ck_plan_hrot_hoisted build
  * the plan with each BaseConv form (fused, FP64 matrix cores, int8 matrix
  * cores; for ck_plan_hmult each with both key passes, CK_OPT_FUSED_KEY_ROWS
- * 1 and 0), run each (one warm-up + 3 timed runs on a private stream,
+ * 1 and 0, all in one chunk, then the fastest of those in 2, 4 and 8 item
+ * chunks), run each (one warm-up + 3 timed runs on a private stream,
  * synchronising the host: plan creation only; the runs write the plan's
  * outputs) and keep the fastest (ck_plan_tuning lists the times); 0
- * (default) uses CK_OPT_BCONV_MFMA. */
+ * (default) uses CK_OPT_BCONV_MFMA and the default chunks. */
 #define CK_OPT_PLAN_AUTOTUNE 2
 /* CK_OPT_FUSED_KEY_ROWS: 1 (default) runs the last ModUp transform stages
  * (forward row stages) and the key inner product of every key switch

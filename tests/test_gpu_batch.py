@@ -66,14 +66,23 @@ def test_plan_hmult_mixed_levels(ck, ctx24, oracle_rns24, key24):
     # (info "form": the plan's key pass)
     assert info["items"] == len(levels)
     tuning = plan.tuning()
+    chunks = 1  # five small items: one chunk by default
     if not tuning:
         assert info["form"] == int(ctx24.fused_key_rows)
-    assert info["launches"] == ((11 if info["bconv_mfma"] else 9) - info["form"])
-    if tuning:  # autotuned: every BaseConv form with both key passes, fastest kept
-        assert sorted((t[0], t[1]) for t in tuning) == [(f, k) for f in range(3) for k in (0, 1)]
+    else:
+        # autotuned: every BaseConv form with both key passes in one chunk,
+        # then the fastest of those in two chunks (five items: at least two
+        # per chunk); the fastest candidate kept
+        one = [t for t in tuning if t[2] == 1]
+        assert sorted((t[0], t[1]) for t in one) == [(f, k) for f in range(3) for k in (0, 1)]
+        best1 = min(one, key=lambda t: t[3])
+        assert [t[:3] for t in tuning if t[2] != 1] == [(best1[0], best1[1], 2)]
         assert min(t[3] for t in tuning) > 0
         best = min(tuning, key=lambda t: t[3])
         assert ("fused", "fp64", "int8")[best[0]] == info["bconv_form"] and best[1] == info["form"]
+        chunks = best[2]
+    # 9 or 11 launches (less one with the fused key pass) per chunk
+    assert info["launches"] == chunks * ((11 if info["bconv_mfma"] else 9) - info["form"])
     for rep in range(2):
         for o in outs:
             o[0].fill_(-1)
