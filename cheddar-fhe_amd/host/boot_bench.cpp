@@ -5,16 +5,19 @@
 // warm-up), checks the result and prints one JSON object: ms per
 // bootstrap, precision bits (-log2 of the max and the mean error), set-up
 // time, the same for 8 ciphertexts with messages uniform in the unit disk
-// bootstrapped by one captured HIP graph (ms per bootstrap over the
-// graph's launch; precision over all 8), and, with profiling on for one
-// extra bootstrap, the summed GPU time per kernel.
+// bootstrapped by one captured HIP graph over `lanes` BootContexts (ms per
+// bootstrap over the graph's launch; precision over all 8; the outputs
+// compared word for word with eager bootstraps of the same inputs), and,
+// with profiling on for one extra bootstrap, the summed GPU time per kernel.
 //
-//   boot_bench <bootparam.json> [runs] [seed] [sse|nosse] [bconv]
+//   boot_bench <bootparam.json> [runs] [seed] [sse|nosse] [bconv] [lanes]
 //
 // sse (default): the file's dense Hamming weight with sparse-secret
 // encapsulation; nosse: the main secret gets the sparse weight. bconv: the
 // CK_OPT_BCONV_MFMA value (-1 default, 0 fused, 1 FP64, 2 int8 matrix
-// cores).
+// cores). lanes (default 4): contexts of the graph, bootstrap i on lane
+// i mod lanes, each lane a stream of its own.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -58,6 +61,7 @@ int main(int argc, char **argv) {
     const Parameter<word> &P = *param;
     const int slots = P.degree_ / 2;
 
+    const int lanes = argc > 6 ? std::max(1, std::atoi(argv[6])) : 4;
     auto t0 = Clock::now();
     auto ctx = BootContext<word>::Create(P, BootParameter(P.max_level_, ncts, nstc));
     if (argc > 5)
@@ -122,9 +126,26 @@ int main(int argc, char **argv) {
       }
     }
 
-    // `batch` bootstraps in one captured graph
+    // `batch` bootstraps in one captured graph over `lanes` contexts (the
+    // first is ctx; the others prepared alike, each with its own workspace)
     const int batch = 8;
-    std::vector<Ciphertext<word>> gin(batch), gout(batch);
+    auto tl = Clock::now();
+    std::vector<std::shared_ptr<BootContext<word>>> extra;
+    std::vector<BootContext<word> *> lane_ctx{ctx.get()};
+    for (int j = 1; j < lanes; j++) {
+      extra.push_back(BootContext<word>::Create(P, BootParameter(P.max_level_, ncts, nstc)));
+      if (argc > 5)
+        Context<word>::Check(ck_context_set_option(extra.back()->Handle(), CK_OPT_BCONV_MFMA,
+                                                   std::atoi(argv[5])));
+      extra.back()->PrepareEvalMod();
+      extra.back()->PrepareEvalSpecialFFT(slots);
+      Ciphertext<word> warm;
+      extra.back()->Boot(warm, ct, ui.GetEvkMap());  // warms the pool for the capture
+      lane_ctx.push_back(extra.back().get());
+    }
+    HipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    const double lanes_setup = Since(tl);
+    std::vector<Ciphertext<word>> gin(batch), gout(batch), geager(batch);
     std::vector<std::vector<Complex>> gmsg(batch, std::vector<Complex>(slots));
     for (int i = 0; i < batch; i++) {
       // uniform in the unit disk: radius sqrt(u), angle 2 pi v
@@ -132,12 +153,14 @@ int main(int argc, char **argv) {
       Plaintext<word> gp;
       ctx->encoder_.Encode(gp, 0, P.GetScale(0), gmsg[i]);
       ui.Encrypt(gin[i], gp);
+      ctx->Boot(geager[i], gin[i], ui.GetEvkMap());
     }
     hipStream_t st;
     HipCheck(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
     double graph_ms = 0;
     size_t graph_nodes = 0;
     double graph_worst = 0, graph_sum = 0;
+    bool graph_same = true;
     {
       std::vector<const Ciphertext<word> *> ip;
       std::vector<Ciphertext<word> *> op;
@@ -146,7 +169,7 @@ int main(int argc, char **argv) {
         op.push_back(&gout[i]);
       }
       HipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
-      BootGraph<word> graph(*ctx, ip, op, ui.GetEvkMap(), st);
+      BootGraph<word> graph(lane_ctx, ip, op, ui.GetEvkMap(), st);
       graph_nodes = graph.Nodes();
       graph.Launch();  // warm-up
       HipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
@@ -154,6 +177,9 @@ int main(int argc, char **argv) {
       for (int r = 0; r < runs; r++) graph.Launch();
       HipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
       graph_ms = 1e3 * Since(t) / (runs * batch);
+      for (int i = 0; i < batch; i++)
+        graph_same = graph_same && gout[i].bx_.CopyToHost() == geager[i].bx_.CopyToHost() &&
+                     gout[i].ax_.CopyToHost() == geager[i].ax_.CopyToHost();
       for (int i = 0; i < batch; i++) {
         Plaintext<word> d;
         ui.Decrypt(d, gout[i]);
@@ -184,6 +210,8 @@ int main(int argc, char **argv) {
                 "\"dense_hamming_weight\": %d, \"sparse_hamming_weight\": %d, "
                 "\"sparse_secret_encapsulation\": %s, \"graph_batch\": %d, "
                 "\"graph_ms_per_bootstrap\": %.3f, \"graph_nodes\": %zu, "
+                "\"graph_lanes\": %d, \"graph_lanes_setup_s\": %.1f, "
+                "\"graph_equal_to_eager\": %s, "
                 "\"graph_precision_bits_max\": %.2f, \"graph_precision_bits_mean\": %.2f, "
                 "\"graph_message\": \"uniform in the unit disk\", \"minks_ms_per_bootstrap\": %.3f, "
                 "\"minks_precision_bits_max\": %.2f, \"kernels\": {",
@@ -191,13 +219,14 @@ int main(int argc, char **argv) {
                 setup, req.size(), slots, P.NPToLevel(out.GetNP()),
                 P.GetDenseHammingWeight(), P.GetSparseHammingWeight(),
                 P.IsUsingSparseSecretEncapsulation() ? "true" : "false", batch, graph_ms,
-                graph_nodes, -std::log2(graph_worst), -std::log2(graph_sum / (batch * slots)),
+                graph_nodes, lanes, lanes_setup, graph_same ? "true" : "false",
+                -std::log2(graph_worst), -std::log2(graph_sum / (batch * slots)),
                 1e3 * minks_total / runs,
                 -std::log2(minks_worst));
     for (int i = 0; i < n; i++)
       std::printf("%s\"%s\": [%.4f, %lld]", i ? ", " : "", &names[64 * i], ms[i], calls[i]);
     std::printf("}}\n");
-    return worst < 1e-3 && graph_worst < 1e-3 && minks_worst < 1e-3 ? 0 : 1;
+    return worst < 1e-3 && graph_worst < 1e-3 && minks_worst < 1e-3 && graph_same ? 0 : 1;
   } catch (const std::exception &e) {
     std::fprintf(stderr, "boot_bench: %s\n", e.what());
     return 1;
