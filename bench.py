@@ -705,7 +705,9 @@ def bench_boot(runs):
     facade's BootContext) on bootparam_40.json, full slots: one ciphertext
     with a message uniform in the unit box (ms per bootstrap over `runs`
     after a warm-up), and 8 ciphertexts with messages uniform in the unit
-    disk in one captured graph (ms per bootstrap, precision over all 8)."""
+    disk in one captured graph over 4 lane contexts, each lane a stream of
+    its own (ms per bootstrap, precision over all 8, outputs equal to eager
+    bootstraps of the same inputs)."""
     exe = os.path.join(REPO, "cheddar-fhe_amd", "boot_bench")
     prm = os.path.join(REPO, "tests", "golden", "params", "bootparam_40.json")
     if not os.path.exists(exe):
@@ -729,8 +731,9 @@ def bench_boot(runs):
     out["launches"] = sum(v[1] for v in by_name.values())
     log(f"bootstrap: {out['ms_per_bootstrap']:.2f} ms, precision {out['precision_bits_max']:.2f} "
         f"bits (max error), {out['precision_bits_mean']:.2f} bits (mean); kernel sum "
-        f"{total:.2f} ms; {out['graph_batch']} in one captured graph: "
-        f"{out['graph_ms_per_bootstrap']:.2f} ms per bootstrap "
+        f"{total:.2f} ms; {out['graph_batch']} in one captured graph over "
+        f"{out['graph_lanes']} lanes: {out['graph_ms_per_bootstrap']:.2f} ms per bootstrap "
+        f"(equal to eager: {out['graph_equal_to_eager']}) "
         f"({out['graph_precision_bits_max']:.2f} bits max, "
         f"{out['graph_precision_bits_mean']:.2f} mean, unit-disk messages); min-KS schedule: "
         f"{out['minks_ms_per_bootstrap']:.2f} ms ({out['minks_precision_bits_max']:.2f} bits)")
