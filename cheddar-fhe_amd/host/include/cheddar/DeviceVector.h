@@ -53,6 +53,20 @@ class DevicePool {
     std::lock_guard<std::mutex> g(m_);
     scope_ = nullptr;
   }
+  Scope *CurrentScope() {
+    std::lock_guard<std::mutex> g(m_);
+    return scope_;
+  }
+  // move s's blocks into scope `into`, or back to the pool when into is null
+  void MergeScope(Scope *s, Scope *into) {
+    if (!into) return ReleaseScope(s);
+    std::lock_guard<std::mutex> g(m_);
+    for (auto &kv : s->free) {
+      auto &dst = into->free[kv.first];
+      dst.insert(dst.end(), kv.second.begin(), kv.second.end());
+    }
+    s->free.clear();
+  }
   // give a finished scope's blocks back to the pool
   void ReleaseScope(Scope *s) {
     std::lock_guard<std::mutex> g(m_);

@@ -13,7 +13,9 @@
 // stream, so the graph holds K independent chains that the GPU overlaps
 // (their launches are latency-bound). Each lane frees into its own pool
 // scope, so no temporary passes between lanes; results are identical to
-// one lane's.
+// one lane's. The lanes capture their EvalMod pairs in order (the two-stream
+// EvalMod of an eager bootstrap measured slower inside a multi-lane graph:
+// 10.4 against 8.8 ms per bootstrap over 4 lanes).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -65,10 +67,18 @@ class BootGraph {
     // streams may still be using: let all device work finish first
     HipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
     std::vector<void *> old(k);
-    for (size_t j = 0; j < k; j++) old[j] = lanes[j]->Stream();
+    std::vector<bool> overlap(k);
+    for (size_t j = 0; j < k; j++) {
+      old[j] = lanes[j]->Stream();
+      overlap[j] = lanes[j]->EvalModOverlap();
+      lanes[j]->SetEvalModOverlap(false);
+    }
     auto restore = [&]() {
       DevicePool::Get().EndScope();
-      for (size_t j = 0; j < k; j++) lanes[j]->SetStream(old[j]);
+      for (size_t j = 0; j < k; j++) {
+        lanes[j]->SetStream(old[j]);
+        lanes[j]->SetEvalModOverlap(overlap[j]);
+      }
     };
     hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed);
     if (e == hipSuccess) {
