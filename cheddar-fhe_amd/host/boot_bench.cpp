@@ -196,7 +196,7 @@ int main(int argc, char **argv) {
 
     // one more bootstrap with per-kernel timing
     // (its EvalMod pair in order, so every launch is on the profiled context)
-    ctx->SetEvalModOverlap(false);
+    ctx->SetPairOverlap(false);
     Context<word>::Check(ck_profile_enable(ctx->Handle(), 1));
     ctx->Boot(out, ct, ui.GetEvkMap());
     constexpr int kMax = 1024;
@@ -205,7 +205,7 @@ int main(int argc, char **argv) {
     std::vector<long long> calls(kMax);
     const int n = ck_profile_report(ctx->Handle(), kMax, names.data(), ms.data(), calls.data());
     Context<word>::Check(ck_profile_enable(ctx->Handle(), 0));
-    ctx->SetEvalModOverlap(true);
+    ctx->SetPairOverlap(true);
 
     std::printf("{\"ms_per_bootstrap\": %.3f, \"runs\": %d, \"precision_bits_max\": %.2f, "
                 "\"precision_bits_mean\": %.2f, \"max_error\": %.3e, \"setup_s\": %.1f, "

@@ -70,14 +70,14 @@ class BootGraph {
     std::vector<bool> overlap(k);
     for (size_t j = 0; j < k; j++) {
       old[j] = lanes[j]->Stream();
-      overlap[j] = lanes[j]->EvalModOverlap();
-      lanes[j]->SetEvalModOverlap(false);
+      overlap[j] = lanes[j]->PairOverlap();
+      lanes[j]->SetPairOverlap(false);
     }
     auto restore = [&]() {
       DevicePool::Get().EndScope();
       for (size_t j = 0; j < k; j++) {
         lanes[j]->SetStream(old[j]);
-        lanes[j]->SetEvalModOverlap(overlap[j]);
+        lanes[j]->SetPairOverlap(overlap[j]);
       }
     };
     hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed);
