@@ -15,7 +15,9 @@
 //   the same bootstrap with the exact-lift dense -> sparse switch;
 // - four bootstraps captured into one HIP graph: three replays, with eager
 //   work and synchronisation between them, each output equal word for word
-//   to the eager bootstrap of the same input;
+//   to the eager bootstrap of the same input (the eager bootstraps run
+//   their pairs of independent steps on two streams, a graph captures them
+//   in order: the comparison also checks that the pairs change nothing);
 // - 2^14 slots (Trace + doubled transforms) and the imaginary-removing
 //   variant at 2^13 slots (the output is the real part of the message).
 //
