@@ -64,9 +64,7 @@ int main(int argc, char **argv) {
     const int lanes = argc > 6 ? std::max(1, std::atoi(argv[6])) : 4;
     auto t0 = Clock::now();
     auto ctx = BootContext<word>::Create(P, BootParameter(P.max_level_, ncts, nstc));
-    if (argc > 5)
-      Context<word>::Check(ck_context_set_option(ctx->Handle(), CK_OPT_BCONV_MFMA,
-                                                 std::atoi(argv[5])));
+    if (argc > 5) ctx->SetOption(CK_OPT_BCONV_MFMA, std::atoi(argv[5]));
     UserInterface<word> ui(ctx, seed);
     ctx->PrepareEvalMod();
     ctx->PrepareEvalSpecialFFT(slots);
@@ -134,9 +132,7 @@ int main(int argc, char **argv) {
     std::vector<BootContext<word> *> lane_ctx{ctx.get()};
     for (int j = 1; j < lanes; j++) {
       extra.push_back(BootContext<word>::Create(P, BootParameter(P.max_level_, ncts, nstc)));
-      if (argc > 5)
-        Context<word>::Check(ck_context_set_option(extra.back()->Handle(), CK_OPT_BCONV_MFMA,
-                                                   std::atoi(argv[5])));
+      if (argc > 5) extra.back()->SetOption(CK_OPT_BCONV_MFMA, std::atoi(argv[5]));
       extra.back()->PrepareEvalMod();
       extra.back()->PrepareEvalSpecialFFT(slots);
       Ciphertext<word> warm;
