@@ -159,10 +159,12 @@ def rocprof_summary(trace_csv):
     return "\n".join(lines)
 
 
-def pmc_traffic(fetch_csv, write_csv):
+def pmc_traffic(fetch_csv, write_csv, scale=None):
     """Per kernel/limbs HBM bytes per launch from FETCH_SIZE and WRITE_SIZE
     (KiB per dispatch), each scaled so that the 256 MiB device copy of
-    --copy-calibration reads 256 MiB and writes 256 MiB."""
+    --copy-calibration reads 256 MiB and writes 256 MiB; for runs without
+    that copy (boot_bench), `scale` gives the (fetch, write) factors of an
+    earlier calibration."""
     def per_kernel(path, counter):
         acc, copy = {}, None
         for r in _read_csv(path):
@@ -181,15 +183,22 @@ def pmc_traffic(fetch_csv, write_csv):
                 n = grid // wg // 16
             elif short in ("key_accum", "tensor", "permute"):
                 n = grid // wg // 64
+            elif short in ("gs_accum", "permute_accum"):
+                n = grid // 32768  # two words per thread
+            elif short == "bs_fused":
+                n = grid // wg // 64  # limbs rounded up to a multiple of 8
             else:
                 continue
             acc.setdefault(f"{short}/{n}", []).append(v)
         return {k: sum(v) / len(v) for k, v in acc.items()}, copy
     fetch, fcopy = per_kernel(fetch_csv, "FETCH_SIZE")
     write, wcopy = per_kernel(write_csv, "WRITE_SIZE")
-    if not fcopy or not wcopy:
-        raise SystemExit("no 256 MiB calibration copy in the counter files")
-    fs, ws = 262144.0 / fcopy, 262144.0 / wcopy
+    if scale:
+        fs, ws = scale
+    elif not fcopy or not wcopy:
+        raise SystemExit("no 256 MiB calibration copy in the counter files (give --pmc-scale)")
+    else:
+        fs, ws = 262144.0 / fcopy, 262144.0 / wcopy
     out = {"fetch_scale": fs, "write_scale": ws, "kernels": {}}
     for k in sorted(set(fetch) & set(write)):
         out["kernels"][k] = {"fetch_kib": fetch[k], "write_kib": write[k],
@@ -457,12 +466,15 @@ def main():
     ap.add_argument("--pmc-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print per-kernel HBM traffic (JSON) from two rocprofv3 --pmc "
                          "runs (FETCH_SIZE, WRITE_SIZE) and exit")
+    ap.add_argument("--pmc-scale", nargs=2, type=float, metavar=("FETCH", "WRITE"),
+                    help="with --pmc-traffic: counter scale factors of an earlier "
+                         "calibration, for runs without the calibration copy")
     a = ap.parse_args()
     if a.rocprof_summary:
         print(rocprof_summary(a.rocprof_summary))
         return
     if a.pmc_traffic:
-        print(json.dumps(pmc_traffic(*a.pmc_traffic), indent=1))
+        print(json.dumps(pmc_traffic(*a.pmc_traffic, scale=a.pmc_scale), indent=1))
         return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported "
